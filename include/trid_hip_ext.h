@@ -1,0 +1,46 @@
+/*
+ * trid_hip_ext.h -- tridsolver_amd extensions with no reference counterpart.
+ *
+ * Runtime control (stream, autotuner, diagnostics) for the drop-in entry
+ * points of trid_hip.h and blktrid_hip.h.  The reference always runs on the
+ * null stream and synchronises the device after every call
+ * (scalar/src/cuda/trid_cuda.cu:442-443); callers that do nothing with these
+ * functions get exactly that behaviour.
+ */
+#ifndef TRIDSOLVER_AMD_TRID_HIP_EXT_H
+#define TRIDSOLVER_AMD_TRID_HIP_EXT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "trid_common.h"
+
+/* Stream on which every entry point of this library enqueues its kernels
+ * (a hipStream_t; NULL = the null stream).  Process-wide. */
+EXTERN_C tridStatus_t tridSetStream(void *stream);
+EXTERN_C void *tridGetStream(void);
+
+/* Autotuner mode: 0 = fixed heuristic table, 1 = time the candidate kernel
+ * configurations on the first call for each (precision, layout class, system
+ * length, batch size) and keep the fastest.  Default 0; the environment
+ * variable TRIDHIP_AUTOTUNE=1 sets 1 at load time. */
+EXTERN_C tridStatus_t tridSetAutotune(int mode);
+
+/* Short text of the last failing call on this thread ("" if none). */
+EXTERN_C const char *tridGetLastError(void);
+EXTERN_C const char *tridGetStatusString(tridStatus_t status);
+
+/* Describes the kernel configuration the last strided-batch call on this
+ * thread used, e.g. "part<double,o0,R8,cps32,spb8>" (for reports and tests). */
+EXTERN_C const char *tridGetLastConfig(void);
+
+/* Forces the next strided-batch calls onto one kernel configuration
+ * (for tests and the autotuner's own sweeps).  kind: 0 = automatic,
+ * 1 = partitioned Thomas with wave/LDS reduced-system PCR, 2 = one thread
+ * per system with a global-memory forward sweep (the fallback).  rows is the
+ * chunk length R of kind 1 (4, 8 or 16; 0 = automatic), cps the chunks per
+ * system (power of two; 0 = automatic), spb the systems per work-group
+ * (0 = automatic). */
+EXTERN_C tridStatus_t tridForceConfig(int kind, int rows, int cps, int spb);
+
+#endif

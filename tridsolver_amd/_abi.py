@@ -1,0 +1,39 @@
+"""ctypes signatures of every symbol declared in include/*.h."""
+from __future__ import annotations
+
+import ctypes as C
+
+P = C.c_void_p
+I = C.c_int
+IP = C.POINTER(C.c_int)
+ST = C.c_int  # tridStatus_t
+
+# name -> (restype, argtypes); kept in step with include/*.h (a CPU test
+# checks that every declared symbol is exported and listed here)
+SIGNATURES = {
+    # trid_hip.h
+    "initTridMultiDimBatchSolve": (None, [I, IP, IP]),
+    "tridSmtsvStridedBatch": (ST, [P, P, P, P, P, I, I, IP, IP, IP, I]),
+    "tridDmtsvStridedBatch": (ST, [P, P, P, P, P, I, I, IP, IP, IP, I]),
+    "tridSmtsvStridedBatchInc": (ST, [P, P, P, P, P, I, I, IP, IP, IP, I]),
+    "tridDmtsvStridedBatchInc": (ST, [P, P, P, P, P, I, I, IP, IP, IP, I]),
+    "tridDmtsvStridedBatchPadded": (ST, [P, IP, P, IP, P, IP, P, IP, P, IP, I, I, IP, IP, I]),
+    "tridDmtsvStridedBatchPaddedInc": (ST, [P, IP, P, IP, P, IP, P, IP, P, IP, I, I, IP, IP, I]),
+    "tridSmtsvStridedBatchPadded": (ST, [P, IP, P, IP, P, IP, P, IP, P, IP, I, I, IP, IP, I]),
+    "tridSmtsvStridedBatchPaddedInc": (ST, [P, IP, P, IP, P, IP, P, IP, P, IP, I, I, IP, IP, I]),
+    # trid_hip_ext.h
+    "tridSetStream": (ST, [P]),
+    "tridGetStream": (P, []),
+    "tridSetAutotune": (ST, [I]),
+    "tridGetLastError": (C.c_char_p, []),
+    "tridGetStatusString": (C.c_char_p, [I]),
+    "tridGetLastConfig": (C.c_char_p, []),
+    "tridForceConfig": (ST, [I, I, I, I]),
+}
+
+
+def declare(lib: C.CDLL) -> None:
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args

@@ -1,0 +1,139 @@
+// trid_runtime.cpp -- stream, error and workspace state of tridsolver_amd.
+#include "trid_runtime.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdlib>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/trid_hip_ext.h"
+
+namespace tridhip {
+
+namespace {
+std::atomic<hipStream_t> g_stream{nullptr};
+std::atomic<int> g_autotune{-1};
+std::mutex g_mu;
+ForcedConfig g_forced;
+thread_local std::string t_error;
+thread_local std::string t_config;
+
+struct DevWs {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+std::vector<DevWs> g_ws;
+}  // namespace
+
+hipStream_t current_stream() { return g_stream.load(); }
+
+tridStatus_t fail(tridStatus_t st, const std::string &msg) {
+  t_error = msg;
+  return st;
+}
+
+void clear_error() { t_error.clear(); }
+
+void set_last_config(const std::string &cfg) { t_config = cfg; }
+
+tridStatus_t check_hip(hipError_t e, const char *what) {
+  if (e == hipSuccess) return TRID_STATUS_SUCCESS;
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    return fail(TRID_STATUS_ALLOC_FAILED, m);
+  if (e == hipErrorNoBinaryForGpu || e == hipErrorInvalidDeviceFunction)
+    return fail(TRID_STATUS_ARCH_MISMATCH, m);
+  return fail(TRID_STATUS_EXECUTION_FAILED, m);
+}
+
+void *workspace(size_t bytes) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1);
+  DevWs &w = g_ws[dev];
+  if (w.bytes >= bytes && w.ptr) return w.ptr;
+  if (w.ptr) {
+    // the old buffer may still be read by queued kernels of this library
+    (void)hipDeviceSynchronize();
+    (void)hipFree(w.ptr);
+    w.ptr = nullptr;
+    w.bytes = 0;
+  }
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  w.ptr = p;
+  w.bytes = bytes;
+  return p;
+}
+
+int autotune_mode() {
+  int m = g_autotune.load();
+  if (m < 0) {
+    const char *e = std::getenv("TRIDHIP_AUTOTUNE");
+    m = (e && e[0] == '1') ? 1 : 0;
+    g_autotune.store(m);
+  }
+  return m;
+}
+
+ForcedConfig forced_config() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return g_forced;
+}
+
+}  // namespace tridhip
+
+using namespace tridhip;
+
+EXTERN_C tridStatus_t tridSetStream(void *stream) {
+  g_stream.store(reinterpret_cast<hipStream_t>(stream));
+  return TRID_STATUS_SUCCESS;
+}
+
+EXTERN_C void *tridGetStream(void) { return g_stream.load(); }
+
+EXTERN_C tridStatus_t tridSetAutotune(int mode) {
+  if (mode != 0 && mode != 1)
+    return fail(TRID_STATUS_INVALID_VALUE, "tridSetAutotune: mode must be 0 or 1");
+  g_autotune.store(mode);
+  return TRID_STATUS_SUCCESS;
+}
+
+EXTERN_C const char *tridGetLastError(void) { return t_error.c_str(); }
+
+EXTERN_C const char *tridGetLastConfig(void) { return t_config.c_str(); }
+
+EXTERN_C const char *tridGetStatusString(tridStatus_t s) {
+  switch (s) {
+    case TRID_STATUS_SUCCESS: return "TRID_STATUS_SUCCESS";
+    case TRID_STATUS_NOT_INITIALIZED: return "TRID_STATUS_NOT_INITIALIZED";
+    case TRID_STATUS_ALLOC_FAILED: return "TRID_STATUS_ALLOC_FAILED";
+    case TRID_STATUS_INVALID_VALUE: return "TRID_STATUS_INVALID_VALUE";
+    case TRID_STATUS_ARCH_MISMATCH: return "TRID_STATUS_ARCH_MISMATCH";
+    case TRID_STATUS_MAPPING_ERROR: return "TRID_STATUS_MAPPING_ERROR";
+    case TRID_STATUS_EXECUTION_FAILED: return "TRID_STATUS_EXECUTION_FAILED";
+    case TRID_STATUS_INTERNAL_ERROR: return "TRID_STATUS_INTERNAL_ERROR";
+    case TRID_STATUS_MATRIX_TYPE_NOT_SUPPORTED: return "TRID_STATUS_MATRIX_TYPE_NOT_SUPPORTED";
+    case TRID_STATUS_ZERO_PIVOT: return "TRID_STATUS_ZERO_PIVOT";
+  }
+  return "unknown tridStatus_t";
+}
+
+EXTERN_C tridStatus_t tridForceConfig(int kind, int rows, int cps, int spb) {
+  if (kind < 0 || kind > 2) return fail(TRID_STATUS_INVALID_VALUE, "tridForceConfig: kind");
+  if (rows != 0 && rows != 4 && rows != 8 && rows != 16)
+    return fail(TRID_STATUS_INVALID_VALUE, "tridForceConfig: rows must be 0, 4, 8 or 16");
+  auto pow2 = [](int v) { return v == 0 || (v > 0 && (v & (v - 1)) == 0); };
+  if (!pow2(cps) || !pow2(spb))
+    return fail(TRID_STATUS_INVALID_VALUE, "tridForceConfig: cps/spb must be powers of two");
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_forced.kind = kind;
+  g_forced.rows = rows;
+  g_forced.cps = cps;
+  g_forced.spb = spb;
+  return TRID_STATUS_SUCCESS;
+}

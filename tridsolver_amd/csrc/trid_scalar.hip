@@ -1,0 +1,495 @@
+// trid_scalar.hip -- strided-batch scalar tridiagonal solves: kernels, the
+// per-call dispatcher and autotuner, and the C entry points of trid_hip.h.
+//
+// Host dispatch replaces tridMultiDimBatchSolve<REAL,INC>
+// (scalar/src/cuda/trid_cuda.cu:191-445): instead of a kernel per solvedim
+// and opts value plus cuBLAS transposes for solvedim 0, every layout goes to
+// one partitioned-Thomas kernel family (trid_part.hpp) whose lane mapping is
+// chosen by the layout class:
+//   solvedim 0  (unit stride along the system)  -> ORDER 0, chunks in lanes
+//   solvedim >= 1 (systems adjacent in memory)  -> ORDER 1, systems in lanes
+// and a one-thread-per-system fallback for shapes outside its limits.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/trid_hip.h"
+#include "../../include/trid_hip_ext.h"
+#include "trid_part.hpp"
+#include "trid_runtime.hpp"
+
+namespace tridhip {
+
+// ---------------------------------------------------------------------------
+// kernels
+
+template <typename T, int INC, int ORDER, int R, int VEC>
+__global__ void __launch_bounds__((R >= 16 && sizeof(T) == 8) ? 512 : 1024)
+    k_part(const T *__restrict__ A, const T *__restrict__ B,
+           const T *__restrict__ C, T *__restrict__ Dd, T *__restrict__ U,
+           SysDesc D, int cps_log2, int spb_log2) {
+  part_block<T, INC, ORDER, R, VEC>(A, B, C, Dd, U, D, cps_log2, spb_log2,
+                                    blockIdx.x);
+}
+
+// Fallback: one thread per system, plain Thomas with the forward sweep's
+// c' and d' in a global workspace laid out [row][system] (coalesced across
+// threads).  Same recurrence as the reference's trid_strided_multidim_kernel
+// (scalar/src/cuda/trid_strided_multidim.hpp:43-87) without the N_MAX cap.
+template <typename T, int INC>
+__global__ void __launch_bounds__(256)
+    k_thomas_gm(const T *__restrict__ A, const T *__restrict__ B,
+                const T *__restrict__ C, T *__restrict__ Dd,
+                T *__restrict__ U, SysDesc D, T *__restrict__ ws) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= D.nsys) return;
+  long long off[5];
+  sys_offsets(D, q, off);
+  const int N = D.N;
+  const long long ns = D.nsys;
+  T *cs = ws;
+  T *ds = ws + (long long)N * ns;
+  const T *pa = A + off[0], *pb = B + off[1], *pc = C + off[2];
+  T *pd = Dd + off[3];
+  T *pu = INC ? U + off[4] : nullptr;
+  const long long sa = D.sstr[0], sb = D.sstr[1], sc = D.sstr[2],
+                  sd = D.sstr[3], su = D.sstr[4];
+  T bb = rcp(pb[0]);
+  T cc = N > 1 ? bb * pc[0] : T(0);
+  T dd = bb * pd[0];
+  cs[q] = cc;
+  ds[q] = dd;
+  for (int i = 1; i < N; i++) {
+    T aa = pa[i * sa];
+    bb = pb[i * sb] - aa * cc;
+    dd = pd[i * sd] - aa * dd;
+    bb = rcp(bb);
+    cc = i < N - 1 ? bb * pc[i * sc] : T(0);
+    dd = bb * dd;
+    cs[i * ns + q] = cc;
+    ds[i * ns + q] = dd;
+  }
+  if (INC)
+    pu[(N - 1) * su] += dd;
+  else
+    pd[(N - 1) * sd] = dd;
+  for (int i = N - 2; i >= 0; i--) {
+    dd = ds[i * ns + q] - cs[i * ns + q] * dd;
+    if (INC)
+      pu[i * su] += dd;
+    else
+      pd[i * sd] = dd;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+namespace {
+
+struct PartCfg {
+  int kind = 1;  // 1 partitioned, 2 fallback
+  int R = 8;
+  int cps_log2 = 0;
+  int spb_log2 = 0;
+  int order = 0;
+  int vec = 0;
+};
+
+int ilog2(long long v) {
+  int l = 0;
+  while ((1LL << l) < v) l++;
+  return l;
+}
+
+template <typename T, int INC, int ORDER, int R, int VEC>
+hipError_t launch_part_t(const PartCfg &cfg, const T *a, const T *b,
+                         const T *c, T *d, T *u, const SysDesc &D,
+                         hipStream_t st) {
+  const int cps = 1 << cfg.cps_log2, spb = 1 << cfg.spb_log2;
+  const int threads = cps * spb;
+  const long long blocks = (D.nsys + spb - 1) / spb;
+  size_t lds = 0;
+  if (ORDER == 1 || cps > kWave) lds = 3 * 2 * (size_t)cps * spb * sizeof(T);
+  hipLaunchKernelGGL((k_part<T, INC, ORDER, R, VEC>), dim3((unsigned)blocks),
+                     dim3(threads), lds, st, a, b, c, d, u, D, cfg.cps_log2,
+                     cfg.spb_log2);
+  return hipGetLastError();
+}
+
+template <typename T, int INC>
+hipError_t launch_part(const PartCfg &cfg, const T *a, const T *b, const T *c,
+                       T *d, T *u, const SysDesc &D, hipStream_t st) {
+#define TRID_CASE(ORD, RR, VV)                                                 \
+  if (cfg.order == ORD && cfg.R == RR && cfg.vec == VV)                        \
+    return launch_part_t<T, INC, ORD, RR, VV>(cfg, a, b, c, d, u, D, st);
+  TRID_CASE(0, 4, 0)
+  TRID_CASE(0, 8, 0)
+  TRID_CASE(0, 16, 0)
+  TRID_CASE(0, 4, 1)
+  TRID_CASE(0, 8, 1)
+  TRID_CASE(0, 16, 1)
+  TRID_CASE(1, 4, 0)
+  TRID_CASE(1, 8, 0)
+  TRID_CASE(1, 16, 0)
+#undef TRID_CASE
+  return hipErrorInvalidValue;
+}
+
+template <typename T, int INC>
+hipError_t launch_fallback(const T *a, const T *b, const T *c, T *d, T *u,
+                           const SysDesc &D, hipStream_t st, T *ws) {
+  const long long blocks = (D.nsys + 255) / 256;
+  hipLaunchKernelGGL((k_thomas_gm<T, INC>), dim3((unsigned)blocks), dim3(256),
+                     0, st, a, b, c, d, u, D, ws);
+  return hipGetLastError();
+}
+
+// Largest system handled by the partitioned kernels (cps <= 1024 chunks).
+constexpr int kMaxCps = 1024;
+
+bool cfg_valid(const PartCfg &cfg, int N) {
+  if (cfg.kind != 1) return true;
+  const long long cps = 1LL << cfg.cps_log2, spb = 1LL << cfg.spb_log2;
+  const long long thr = cps * spb;
+  if (thr < kWave || thr > 1024) return false;
+  if (cfg.R >= 16 && thr > 512) return false;  // register budget of R=16
+  if (cps * cfg.R < N) return false;
+  return true;
+}
+
+// Threads per block preferred for a given chunk count.
+PartCfg make_cfg(int order, int R, int N, int vec, int threads) {
+  PartCfg c;
+  c.kind = 1;
+  c.order = order;
+  c.R = R;
+  c.vec = vec;
+  long long chunks = (std::max(N, 1) + R - 1) / R;
+  c.cps_log2 = ilog2(chunks);
+  int cps = 1 << c.cps_log2;
+  int spb = std::max(1, threads / cps);
+  if (order == 1) spb = std::min(spb, 64);
+  c.spb_log2 = ilog2(spb);
+  return c;
+}
+
+std::string cfg_name(const PartCfg &c, size_t tsize) {
+  char buf[128];
+  if (c.kind == 2) {
+    snprintf(buf, sizeof buf, "fallback<%s>", tsize == 8 ? "double" : "float");
+  } else {
+    snprintf(buf, sizeof buf, "part<%s,o%d,R%d,cps%d,spb%d%s>",
+             tsize == 8 ? "double" : "float", c.order, c.R, 1 << c.cps_log2,
+             1 << c.spb_log2, c.vec ? ",vec" : "");
+  }
+  return buf;
+}
+
+std::vector<PartCfg> candidates(int order, int N, int vec, size_t tsize) {
+  std::vector<PartCfg> out;
+  const int Rs[3] = {4, 8, 16};
+  const int thr[3] = {256, 512, 1024};
+  for (int R : Rs)
+    for (int t : thr) {
+      PartCfg c = make_cfg(order, R, N, vec && (R * (int)tsize) % 16 == 0, t);
+      if (!cfg_valid(c, N)) continue;
+      bool dup = false;
+      for (auto &o : out)
+        dup |= (o.R == c.R && o.cps_log2 == c.cps_log2 &&
+                o.spb_log2 == c.spb_log2 && o.vec == c.vec);
+      if (!dup) out.push_back(c);
+    }
+  return out;
+}
+
+PartCfg heuristic(int order, int N, int vec, size_t tsize) {
+  // preferred chunk lengths: FP64 8 rows, FP32 16 rows (register budget)
+  const int pref[3] = {tsize == 8 ? 8 : 16, tsize == 8 ? 16 : 8, 4};
+  for (int R : pref) {
+    int v = vec && (R * (int)tsize) % 16 == 0;
+    PartCfg c = make_cfg(order, R, N, v, order == 0 ? 256 : 512);
+    if (cfg_valid(c, N)) return c;
+    c = make_cfg(order, R, N, v, 1024);
+    if (cfg_valid(c, N)) return c;
+  }
+  for (int R : pref) {
+    for (int t : {64, 128, 256, 512, 1024}) {
+      int v = vec && (R * (int)tsize) % 16 == 0;
+      PartCfg c = make_cfg(order, R, N, v, t);
+      if (cfg_valid(c, N)) return c;
+    }
+  }
+  PartCfg f;
+  f.kind = 2;
+  return f;
+}
+
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int, long long, int>, PartCfg> g_tuned;
+
+// Extent (in elements) of an array with the given dims/pads.
+long long extent(int ndim, const int *dims, const int *pads) {
+  long long e = 0, cum = 1;
+  for (int k = 0; k < ndim; k++) {
+    if (dims[k] == 0) return 0;
+    e += (long long)(dims[k] - 1) * cum;
+    cum *= pads[k];
+  }
+  return e + 1;
+}
+
+template <typename T, int INC>
+tridStatus_t run_cfg(const PartCfg &cfg, const T *a, const T *b, const T *c,
+                     T *d, T *u, const SysDesc &D, hipStream_t st) {
+  hipError_t e;
+  if (cfg.kind == 2) {
+    size_t bytes = 2 * (size_t)D.N * (size_t)D.nsys * sizeof(T);
+    T *ws = static_cast<T *>(workspace(bytes));
+    if (!ws) return fail(TRID_STATUS_ALLOC_FAILED, "workspace allocation failed");
+    e = launch_fallback<T, INC>(a, b, c, d, u, D, st, ws);
+  } else {
+    e = launch_part<T, INC>(cfg, a, b, c, d, u, D, st);
+  }
+  return check_hip(e, "kernel launch");
+}
+
+template <typename T, int INC>
+PartCfg autotune(int order, int vec, const T *a, const T *b, const T *c, T *d,
+                 T *u, const SysDesc &D, long long d_extent, hipStream_t st) {
+  std::vector<PartCfg> cands = candidates(order, D.N, vec, sizeof(T));
+  PartCfg best = heuristic(order, D.N, vec, sizeof(T));
+  if (cands.size() < 2) return best;
+  // scratch copy of the written array so that tuning leaves the caller's
+  // data untouched
+  T *scratch = static_cast<T *>(workspace((size_t)d_extent * sizeof(T)));
+  if (!scratch) return best;
+  T *src = INC ? u : d;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return best;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return best;
+  }
+  float best_ms = 1e30f;
+  for (const PartCfg &cf : cands) {
+    if (hipMemcpyAsync(scratch, src, (size_t)d_extent * sizeof(T),
+                       hipMemcpyDeviceToDevice, st) != hipSuccess)
+      break;
+    T *dd = INC ? d : scratch;
+    T *uu = INC ? scratch : u;
+    if (launch_part<T, INC>(cf, a, b, c, dd, uu, D, st) != hipSuccess) continue;
+    (void)hipEventRecord(e0, st);
+    const int reps = 3;
+    for (int r = 0; r < reps; r++) (void)launch_part<T, INC>(cf, a, b, c, dd, uu, D, st);
+    (void)hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess) break;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = cf;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipGetLastError();
+  return best;
+}
+
+bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+template <typename T, int INC>
+tridStatus_t solve(const T *a, const int *a_pads, const T *b,
+                   const int *b_pads, const T *c, const int *c_pads, T *d,
+                   const int *d_pads, T *u, const int *u_pads, int ndim,
+                   int solvedim, const int *dims, const int *opts, int sync) {
+  (void)opts;  // accepted for compatibility; the kernel choice is automatic
+  clear_error();
+  if (!dims) return fail(TRID_STATUS_INVALID_VALUE, "dims is NULL");
+  if (ndim < 1 || ndim > MAXDIM)
+    return fail(TRID_STATUS_INVALID_VALUE, "ndim must be in [1, 8]");
+  if (solvedim < 0 || solvedim >= ndim)
+    return fail(TRID_STATUS_INVALID_VALUE, "solvedim must be in [0, ndim)");
+  const int *pads[5] = {a_pads ? a_pads : dims, b_pads ? b_pads : dims,
+                        c_pads ? c_pads : dims, d_pads ? d_pads : dims,
+                        u_pads ? u_pads : (d_pads ? d_pads : dims)};
+  for (int k = 0; k < ndim; k++) {
+    if (dims[k] < 0) return fail(TRID_STATUS_INVALID_VALUE, "negative dims");
+    for (int t = 0; t < 5; t++)
+      if (pads[t][k] < dims[k])
+        return fail(TRID_STATUS_INVALID_VALUE, "pads[i] < dims[i]");
+  }
+  SysDesc D;
+  memset(&D, 0, sizeof D);
+  D.N = dims[solvedim];
+  D.nsys = 1;
+  for (int k = 0; k < ndim; k++)
+    if (k != solvedim) {
+      D.osize[D.nother++] = dims[k];
+      D.nsys *= dims[k];
+    }
+  if (D.N == 0 || D.nsys == 0) return TRID_STATUS_SUCCESS;
+  if (!a || !b || !c || !d)
+    return fail(TRID_STATUS_INVALID_VALUE, "a, b, c and d must be non-NULL");
+  if (INC && !u) return fail(TRID_STATUS_INVALID_VALUE, "u is NULL in an Inc call");
+  for (int t = 0; t < 5; t++) {
+    long long cum = 1;
+    int o = 0;
+    for (int k = 0; k < ndim; k++) {
+      if (k == solvedim)
+        D.sstr[t] = cum;
+      else
+        D.ostr[t][o++] = cum;
+      cum *= pads[t][k];
+    }
+  }
+  D.small_index = D.nsys < (1LL << 31);
+
+  const int order = solvedim == 0 ? 0 : 1;
+  // vector loads along the system need unit stride and 16-byte aligned chunks
+  constexpr int E = 16 / sizeof(T);
+  int vec = order == 0;
+  const void *ptrs[5] = {a, b, c, d, u};
+  for (int t = 0; t < 5 && vec; t++) {
+    if (t == 4 && !INC) break;
+    if (!aligned16(ptrs[t])) vec = 0;
+    for (int k = 0; k < D.nother; k++)
+      if (D.ostr[t][k] % E != 0) vec = 0;
+  }
+
+  hipStream_t st = current_stream();
+  PartCfg cfg;
+  ForcedConfig fc = forced_config();
+  if (fc.kind == 2) {
+    cfg.kind = 2;
+  } else if (fc.kind == 1) {
+    int R = fc.rows ? fc.rows : (sizeof(T) == 8 ? 8 : 16);
+    cfg = make_cfg(order, R, D.N, vec && (R * (int)sizeof(T)) % 16 == 0,
+                   order == 0 ? 256 : 512);
+    if (fc.cps) cfg.cps_log2 = ilog2(fc.cps);
+    if (fc.spb) cfg.spb_log2 = ilog2(fc.spb);
+    if (!cfg_valid(cfg, D.N))
+      return fail(TRID_STATUS_INVALID_VALUE, "forced configuration does not fit this shape");
+  } else if (autotune_mode() == 1) {
+    auto key = std::make_tuple((int)sizeof(T), INC, order, D.N, D.nsys, vec);
+    bool have = false;
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tuned.find(key);
+      if (it != g_tuned.end()) {
+        cfg = it->second;
+        have = true;
+      }
+    }
+    if (!have) {
+      long long ext = extent(ndim, dims, INC ? pads[4] : pads[3]);
+      cfg = autotune<T, INC>(order, vec, a, b, c, d, u, D, ext, st);
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tuned[key] = cfg;
+    }
+  } else {
+    cfg = heuristic(order, D.N, vec, sizeof(T));
+  }
+  set_last_config(cfg_name(cfg, sizeof(T)));
+  tridStatus_t s = run_cfg<T, INC>(cfg, a, b, c, d, u, D, st);
+  if (s != TRID_STATUS_SUCCESS) return s;
+  if (sync) return check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+  return TRID_STATUS_SUCCESS;
+}
+
+}  // namespace
+}  // namespace tridhip
+
+using tridhip::solve;
+
+EXTERN_C void initTridMultiDimBatchSolve(int ndim, int *dims, int *pads) {
+  (void)ndim;
+  (void)dims;
+  (void)pads;
+}
+
+EXTERN_C tridStatus_t tridSmtsvStridedBatch(const float *a, const float *b,
+                                            const float *c, float *d, float *u,
+                                            int ndim, int solvedim, int *dims,
+                                            int *pads, int *opts, int sync) {
+  (void)u;
+  return solve<float, 0>(a, pads, b, pads, c, pads, d, pads, nullptr, pads,
+                         ndim, solvedim, dims, opts, sync);
+}
+
+EXTERN_C tridStatus_t tridDmtsvStridedBatch(const double *a, const double *b,
+                                            const double *c, double *d,
+                                            double *u, int ndim, int solvedim,
+                                            int *dims, int *pads, int *opts,
+                                            int sync) {
+  (void)u;
+  return solve<double, 0>(a, pads, b, pads, c, pads, d, pads, nullptr, pads,
+                          ndim, solvedim, dims, opts, sync);
+}
+
+EXTERN_C tridStatus_t tridSmtsvStridedBatchInc(const float *a, const float *b,
+                                               const float *c, float *d,
+                                               float *u, int ndim,
+                                               int solvedim, int *dims,
+                                               int *pads, int *opts,
+                                               int sync) {
+  return solve<float, 1>(a, pads, b, pads, c, pads, d, pads, u, pads, ndim,
+                         solvedim, dims, opts, sync);
+}
+
+EXTERN_C tridStatus_t tridDmtsvStridedBatchInc(const double *a,
+                                               const double *b,
+                                               const double *c, double *d,
+                                               double *u, int ndim,
+                                               int solvedim, int *dims,
+                                               int *pads, int *opts,
+                                               int sync) {
+  return solve<double, 1>(a, pads, b, pads, c, pads, d, pads, u, pads, ndim,
+                          solvedim, dims, opts, sync);
+}
+
+EXTERN_C tridStatus_t tridDmtsvStridedBatchPadded(
+    const double *a, const int *a_pads, const double *b, const int *b_pads,
+    const double *c, const int *c_pads, double *d, const int *d_pads,
+    double *u, const int *u_pads, int ndim, int solvedim, int *dims, int *opts,
+    int sync) {
+  (void)u;
+  return solve<double, 0>(a, a_pads, b, b_pads, c, c_pads, d, d_pads, nullptr,
+                          u_pads, ndim, solvedim, dims, opts, sync);
+}
+
+EXTERN_C tridStatus_t tridDmtsvStridedBatchPaddedInc(
+    const double *a, const int *a_pads, const double *b, const int *b_pads,
+    const double *c, const int *c_pads, double *d, const int *d_pads,
+    double *u, const int *u_pads, int ndim, int solvedim, int *dims, int *opts,
+    int sync) {
+  return solve<double, 1>(a, a_pads, b, b_pads, c, c_pads, d, d_pads, u,
+                          u_pads, ndim, solvedim, dims, opts, sync);
+}
+
+EXTERN_C tridStatus_t tridSmtsvStridedBatchPadded(
+    const float *a, const int *a_pads, const float *b, const int *b_pads,
+    const float *c, const int *c_pads, float *d, const int *d_pads, float *u,
+    const int *u_pads, int ndim, int solvedim, int *dims, int *opts,
+    int sync) {
+  (void)u;
+  return solve<float, 0>(a, a_pads, b, b_pads, c, c_pads, d, d_pads, nullptr,
+                         u_pads, ndim, solvedim, dims, opts, sync);
+}
+
+EXTERN_C tridStatus_t tridSmtsvStridedBatchPaddedInc(
+    const float *a, const int *a_pads, const float *b, const int *b_pads,
+    const float *c, const int *c_pads, float *d, const int *d_pads, float *u,
+    const int *u_pads, int ndim, int solvedim, int *dims, int *opts,
+    int sync) {
+  return solve<float, 1>(a, a_pads, b, b_pads, c, c_pads, d, d_pads, u,
+                         u_pads, ndim, solvedim, dims, opts, sync);
+}
