@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--autotune", type=int, default=1)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (profiler calibration)")
     return ap.parse_args()
 
 
@@ -130,6 +132,15 @@ def main():
     import tridsolver_amd as ta
     from tridsolver_amd import api
 
+    if args.calibrate_copy:
+        # reference kernel for rocprofv3 --pmc: reads and writes exactly 1 GiB
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": 1 << 30, "launches": 5}))
+        return
     ta.set_stream(None)  # null stream == torch's default stream
     ta.set_autotune(args.autotune)
     W = plain_workload(args, torch, np, api, ta)

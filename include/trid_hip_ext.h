@@ -43,4 +43,10 @@ EXTERN_C const char *tridGetLastConfig(void);
  * (0 = automatic). */
 EXTERN_C tridStatus_t tridForceConfig(int kind, int rows, int cps, int spb);
 
+/* Diagnostic: y[i] = the solvers' reciprocal of x[i] for n device values
+ * (double when is_double, else float).  The solvers form every pivot
+ * reciprocal with this routine; tests check its error bound. */
+EXTERN_C tridStatus_t tridDebugReciprocal(const void *x, void *y, long long n,
+                                          int is_double);
+
 #endif

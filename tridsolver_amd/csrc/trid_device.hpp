@@ -12,8 +12,8 @@ constexpr int kWave = 64;
 
 // Description of a strided batch: every system has length N; system q's
 // element i of array t (0=a, 1=b, 2=c, 3=d, 4=u) sits at
-// off_t(q) + i * sstr[t], where off_t(q) decomposes q over the non-solve
-// dimensions (fastest first) with per-array strides.
+// off_t(q) + i * sstr[t] (in elements), where off_t(q) decomposes q over the
+// non-solve dimensions (fastest first) with per-array strides.
 struct SysDesc {
   long long nsys;
   long long sstr[5];
@@ -21,11 +21,14 @@ struct SysDesc {
   int osize[kMaxDim];
   int nother;
   int N;
-  int small_index;  // 1 when nsys and every osize fit 32-bit division
+  int small_index;  // 1 when nsys fits 32-bit division
 };
 
+// Element offsets of system q in the five arrays, in index type I (uint32_t
+// when the host has checked that every array's extent fits, else int64).
+template <typename I>
 __device__ __forceinline__ void sys_offsets(const SysDesc &D, long long q,
-                                            long long off[5]) {
+                                            I off[5]) {
 #pragma unroll
   for (int t = 0; t < 5; t++) off[t] = 0;
   if (D.small_index) {
@@ -35,7 +38,7 @@ __device__ __forceinline__ void sys_offsets(const SysDesc &D, long long q,
       uint32_t ik = qq % sz;
       qq /= sz;
 #pragma unroll
-      for (int t = 0; t < 5; t++) off[t] += (long long)ik * D.ostr[t][k];
+      for (int t = 0; t < 5; t++) off[t] += (I)ik * (I)D.ostr[t][k];
     }
   } else {
     for (int k = 0; k < D.nother; k++) {
@@ -43,15 +46,36 @@ __device__ __forceinline__ void sys_offsets(const SysDesc &D, long long q,
       long long ik = q % sz;
       q /= sz;
 #pragma unroll
-      for (int t = 0; t < 5; t++) off[t] += ik * D.ostr[t][k];
+      for (int t = 0; t < 5; t++) off[t] += (I)ik * (I)D.ostr[t][k];
     }
   }
 }
 
-// IEEE reciprocal (correctly rounded division).
-template <typename T>
-__device__ __forceinline__ T rcp(T x) {
-  return T(1) / x;
+// Reciprocal.  Default: the hardware estimate (v_rcp_f64 / v_rcp_f32)
+// refined by Newton-Raphson steps y <- y + y(1 - x y) written with fused
+// multiply-adds (two steps in FP64, one in FP32), which leaves the result
+// within about one ulp of 1/x for finite non-zero x of normal range.  A zero
+// or infinite x yields NaN (a singular pivot).  Define TRIDHIP_IEEE_DIV for
+// correctly rounded 1/x (the compiler's full division sequence) instead.
+__device__ __forceinline__ double rcp(double x) {
+#ifdef TRIDHIP_IEEE_DIV
+  return 1.0 / x;
+#else
+  double y = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, y, 1.0);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-x, y, 1.0);
+  return __builtin_fma(y, e, y);
+#endif
+}
+__device__ __forceinline__ float rcp(float x) {
+#ifdef TRIDHIP_IEEE_DIV
+  return 1.0f / x;
+#else
+  float y = __builtin_amdgcn_rcpf(x);
+  float e = __builtin_fmaf(-x, y, 1.0f);
+  return __builtin_fmaf(y, e, y);
+#endif
 }
 
 // Lane permute through the LDS crossbar (ds_bpermute): returns v of lane

@@ -4,12 +4,13 @@
 // chunk of one system and keeps all of it in registers.  Each lane runs a
 // modified Thomas sweep over its chunk that expresses every row through the
 // chunk's first and last unknowns, which leaves two rows per chunk coupled
-// to the neighbouring chunks.  Those 2*C rows (C = chunks per system) form a
-// tridiagonal system again; it is solved by parallel cyclic reduction (PCR),
-// across lanes with ds_bpermute when all chunks of a system are in one wave,
-// through LDS otherwise.  Each lane then recovers its interior rows from the
-// two solved boundary unknowns.  Global memory is read once (a, b, c, d) and
-// written once (d, or u for the Inc form): no c'/d' scratch traffic.
+// to the neighbouring chunks.  Eliminating each chunk's first unknown from
+// those rows leaves one row per chunk (in the chunks' last unknowns), again
+// tridiagonal; it is solved by parallel cyclic reduction (PCR) across lanes
+// -- with ds_bpermute when all chunks of a system are in one wave, through
+// LDS otherwise.  Each lane then recovers its first unknown and its interior
+// rows.  Global memory is read once (a, b, c, d) and written once (d, or u
+// for the Inc form): no c'/d' scratch traffic.
 //
 // The reference solves the same systems with the plain Thomas recurrence
 // (scalar/src/cuda/trid_strided_multidim.hpp:43-87, c2/d2 in local memory)
@@ -90,6 +91,23 @@ __device__ __forceinline__ void chunk_expand(const T (&a)[R], const T (&c)[R],
   for (int i = 1; i < R - 1; i++) y[i] = d[i] - a[i] * y0 - c[i] * yl;
 }
 
+// Row of chunk k in the last-unknown system, given the chunk's own two
+// reduced rows f, l and the next chunk's first row fn (zero row if none):
+// substituting f_k = f.d - f.a l_{k-1} - f.c l_k and the same for f_{k+1}
+// into l's row gives  alpha l_{k-1} + l_k + gamma l_{k+1} = delta.
+template <typename T>
+__device__ __forceinline__ Row<T> eliminate_first(const Row<T> &f,
+                                                  const Row<T> &l,
+                                                  const Row<T> &fn) {
+  T bn = T(1) - l.a * f.c - l.c * fn.a;
+  T inv = rcp(bn);
+  Row<T> r;
+  r.a = -l.a * f.a * inv;
+  r.c = -l.c * fn.c * inv;
+  r.d = (l.d - l.a * f.d - l.c * fn.d) * inv;
+  return r;
+}
+
 template <typename T>
 __device__ __forceinline__ Row<T> row_from_lane(const Row<T> &v, int src,
                                                 bool valid) {
@@ -106,83 +124,124 @@ __device__ __forceinline__ Row<T> row_from_lane(const Row<T> &v, int src,
   return o;
 }
 
-// PCR over the 2*cps reduced rows of a system whose cps chunks sit in
-// consecutive lanes of one wave (cps a power of two <= 64).  k is the lane's
-// chunk index, lane its lane id.  Rows are ordered f_0, l_0, f_1, l_1, ...
-// On return f.d and l.d hold the chunk's first and last unknowns.
+// Reduced solve for a system whose cps chunks sit in consecutive lanes of
+// one wave (cps a power of two <= 64).  k: the lane's chunk index.
+// Returns the chunk's first (xf) and last (xl) unknowns.
 template <typename T>
-__device__ __forceinline__ void pcr_wave(Row<T> &f, Row<T> &l, int k, int cps,
-                                         int lane) {
-  {
-    Row<T> lm = row_from_lane(l, lane - 1, k > 0);
-    Row<T> fp = row_from_lane(f, lane + 1, k + 1 < cps);
-    Row<T> nf = pcr_update(f, lm, l);
-    Row<T> nl = pcr_update(l, f, fp);
-    f = nf;
-    l = nl;
-  }
+__device__ __forceinline__ void reduced_wave(const Row<T> &f, const Row<T> &l,
+                                             int k, int cps, int lane, T &xf,
+                                             T &xl) {
+  Row<T> fn = row_from_lane(f, lane + 1, k + 1 < cps);
+  Row<T> r = eliminate_first(f, l, fn);
   for (int s = 1; s < cps; s <<= 1) {
-    Row<T> fm = row_from_lane(f, lane - s, k >= s);
-    Row<T> fp = row_from_lane(f, lane + s, k + s < cps);
-    Row<T> lm = row_from_lane(l, lane - s, k >= s);
-    Row<T> lp = row_from_lane(l, lane + s, k + s < cps);
-    f = pcr_update(f, fm, fp);
-    l = pcr_update(l, lm, lp);
+    Row<T> m = row_from_lane(r, lane - s, k >= s);
+    Row<T> p = row_from_lane(r, lane + s, k + s < cps);
+    r = pcr_update(r, m, p);
+  }
+  xl = r.d;
+  T xm = lane_read(xl, k > 0 ? lane - 1 : lane);
+  if (k == 0) xm = T(0);
+  xf = f.d - f.a * xm - f.c * xl;
+}
+
+// Same through LDS for chunks spread over several waves.  Entry (chunk k,
+// system slot s) lives at k * spb + s of each of two ping-pong buffers of
+// 3 * cps * spb values (one barrier per exchange).  All threads call it.
+template <typename T>
+__device__ __forceinline__ void reduced_lds(const Row<T> &f, const Row<T> &l,
+                                            int k, int cps, int slot, int spb,
+                                            T *sm, T &xf, T &xl) {
+  const int n = cps * spb;
+  const int me = k * spb + slot;
+  int par = 0;
+  auto publish = [&](const Row<T> &v) {
+    T *b = sm + par * 3 * n;
+    b[me] = v.a;
+    b[n + me] = v.c;
+    b[2 * n + me] = v.d;
+  };
+  auto fetch = [&](int kk, bool valid) {
+    Row<T> o{T(0), T(0), T(0)};
+    if (valid) {
+      const T *b = sm + par * 3 * n;
+      int i = kk * spb + slot;
+      o.a = b[i];
+      o.c = b[n + i];
+      o.d = b[2 * n + i];
+    }
+    return o;
+  };
+  publish(f);
+  __syncthreads();
+  Row<T> fn = fetch(k + 1, k + 1 < cps);
+  par ^= 1;
+  Row<T> r = eliminate_first(f, l, fn);
+  for (int s = 1; s < cps; s <<= 1) {
+    publish(r);
+    __syncthreads();
+    Row<T> m = fetch(k - s, k >= s);
+    Row<T> p = fetch(k + s, k + s < cps);
+    par ^= 1;
+    r = pcr_update(r, m, p);
+  }
+  xl = r.d;
+  {
+    T *b = sm + par * 3 * n;
+    b[me] = xl;
+    __syncthreads();
+    T xm = k > 0 ? b[me - spb] : T(0);
+    xf = f.d - f.a * xm - f.c * xl;
   }
 }
 
-// Same PCR through LDS for chunks spread over several waves.  Row rho of
-// system slot s lives at rho * spb + s.  All threads of the block call it.
+// Streaming (non-temporal) global accesses: every element of a, b, c, d
+// and u is touched exactly once per solve, so the loads and stores carry the
+// non-temporal hint.  On MI355X this lifts a read-4/write-1 stream from
+// about 68% to 79% of peak HBM bandwidth.
 template <typename T>
-__device__ __forceinline__ void pcr_lds(Row<T> &f, Row<T> &l, int k, int cps,
-                                        int slot, int spb, T *sa, T *sc,
-                                        T *sd) {
-  const int M = 2 * cps;
-  const int rf = 2 * k, rl = 2 * k + 1;
-  for (int h = 1; h < M; h <<= 1) {
-    sa[rf * spb + slot] = f.a;
-    sc[rf * spb + slot] = f.c;
-    sd[rf * spb + slot] = f.d;
-    sa[rl * spb + slot] = l.a;
-    sc[rl * spb + slot] = l.c;
-    sd[rl * spb + slot] = l.d;
-    __syncthreads();
-    Row<T> z{T(0), T(0), T(0)};
-    Row<T> fm = z, fp = z, lm = z, lp = z;
-    if (rf - h >= 0) {
-      int i = (rf - h) * spb + slot;
-      fm = Row<T>{sa[i], sc[i], sd[i]};
-    }
-    if (rf + h < M) {
-      int i = (rf + h) * spb + slot;
-      fp = Row<T>{sa[i], sc[i], sd[i]};
-    }
-    if (rl - h >= 0) {
-      int i = (rl - h) * spb + slot;
-      lm = Row<T>{sa[i], sc[i], sd[i]};
-    }
-    if (rl + h < M) {
-      int i = (rl + h) * spb + slot;
-      lp = Row<T>{sa[i], sc[i], sd[i]};
-    }
-    __syncthreads();
-    f = pcr_update(f, fm, fp);
-    l = pcr_update(l, lm, lp);
-  }
+__device__ __forceinline__ T ld_nt(const T *p) {
+  return __builtin_nontemporal_load(p);
+}
+template <typename T>
+__device__ __forceinline__ void st_nt(T *p, T v) {
+  __builtin_nontemporal_store(v, p);
 }
 
 template <typename T>
 struct Vec16;
 template <>
 struct Vec16<double> {
-  using type = double2;
+  typedef double type __attribute__((ext_vector_type(2)));
   static constexpr int n = 2;
 };
 template <>
 struct Vec16<float> {
-  using type = float4;
+  typedef float type __attribute__((ext_vector_type(4)));
   static constexpr int n = 4;
 };
+
+// Global access at an element offset held in index type I.  With I =
+// uint32_t the byte offset is formed in 32 bits (the host guarantees every
+// array fits in 4 GiB), which lets the compiler use the scalar-base +
+// 32-bit vector-offset addressing form.
+template <typename T, typename I>
+__device__ __forceinline__ const T *at(const T *base, I off) {
+  if constexpr (sizeof(I) == 4) {
+    return reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) +
+                                       (uint32_t)(off * (I)sizeof(T)));
+  } else {
+    return base + off;
+  }
+}
+template <typename T, typename I>
+__device__ __forceinline__ T *at(T *base, I off) {
+  if constexpr (sizeof(I) == 4) {
+    return reinterpret_cast<T *>(reinterpret_cast<char *>(base) +
+                                 (uint32_t)(off * (I)sizeof(T)));
+  } else {
+    return base + off;
+  }
+}
 
 template <typename T, int R>
 __device__ __forceinline__ void load_chunk_vec(const T *__restrict__ p,
@@ -193,16 +252,9 @@ __device__ __forceinline__ void load_chunk_vec(const T *__restrict__ p,
   const V *pv = reinterpret_cast<const V *>(p);
 #pragma unroll
   for (int v = 0; v < R / E; v++) {
-    V t = pv[v];
-    if constexpr (E == 2) {
-      x[2 * v] = t.x;
-      x[2 * v + 1] = t.y;
-    } else {
-      x[4 * v] = t.x;
-      x[4 * v + 1] = t.y;
-      x[4 * v + 2] = t.z;
-      x[4 * v + 3] = t.w;
-    }
+    V t = ld_nt(pv + v);
+#pragma unroll
+    for (int e = 0; e < E; e++) x[E * v + e] = t[e];
   }
 }
 
@@ -215,17 +267,68 @@ __device__ __forceinline__ void store_chunk_vec(T *__restrict__ p,
 #pragma unroll
   for (int v = 0; v < R / E; v++) {
     V t;
-    if constexpr (E == 2) {
-      t.x = x[2 * v];
-      t.y = x[2 * v + 1];
-    } else {
-      t.x = x[4 * v];
-      t.y = x[4 * v + 1];
-      t.z = x[4 * v + 2];
-      t.w = x[4 * v + 3];
-    }
-    pv[v] = t;
+#pragma unroll
+    for (int e = 0; e < E; e++) t[e] = x[E * v + e];
+    st_nt(pv + v, t);
   }
+}
+
+// Wave-staged access for contiguous systems (ORDER 0): the wave's 64*R
+// elements of one array form one contiguous span.  Global memory is moved
+// in 16-byte pieces with consecutive lanes on consecutive pieces (each
+// instruction covers 1 KiB without gaps), and the lane <-> chunk transpose
+// goes through a wave-private LDS buffer of 64*R elements.  LDS operations
+// of one wave execute in order, so no barrier is needed; wave_barrier only
+// keeps the compiler from reordering the LDS accesses.
+template <typename T, int R>
+__device__ __forceinline__ void load_chunk_staged(const T *__restrict__ span,
+                                                  T *__restrict__ buf,
+                                                  int lane, T (&x)[R]) {
+  using V = typename Vec16<T>::type;
+  constexpr int E = Vec16<T>::n;
+  const V *g = reinterpret_cast<const V *>(span);
+  V *s = reinterpret_cast<V *>(buf);
+  V t[R / E];
+#pragma unroll
+  for (int v = 0; v < R / E; v++) t[v] = ld_nt(g + v * kWave + lane);
+#pragma unroll
+  for (int v = 0; v < R / E; v++) s[v * kWave + lane] = t[v];
+  __builtin_amdgcn_wave_barrier();
+  const V *mine = reinterpret_cast<const V *>(buf + lane * R);
+#pragma unroll
+  for (int v = 0; v < R / E; v++) {
+    V w = mine[v];
+#pragma unroll
+    for (int e = 0; e < E; e++) x[E * v + e] = w[e];
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void store_chunk_staged(T *__restrict__ span,
+                                                   T *__restrict__ buf,
+                                                   int lane, const T (&x)[R],
+                                                   bool add) {
+  using V = typename Vec16<T>::type;
+  constexpr int E = Vec16<T>::n;
+  V *mine = reinterpret_cast<V *>(buf + lane * R);
+#pragma unroll
+  for (int v = 0; v < R / E; v++) {
+    V w;
+#pragma unroll
+    for (int e = 0; e < E; e++) w[e] = x[E * v + e];
+    mine[v] = w;
+  }
+  __builtin_amdgcn_wave_barrier();
+  V *g = reinterpret_cast<V *>(span);
+  const V *s = reinterpret_cast<const V *>(buf);
+#pragma unroll
+  for (int v = 0; v < R / E; v++) {
+    V w = s[v * kWave + lane];
+    if (add) w += ld_nt(g + v * kWave + lane);
+    st_nt(g + v * kWave + lane, w);
+  }
+  __builtin_amdgcn_wave_barrier();
 }
 
 // Block-wide partitioned solve of a strided batch.
@@ -233,10 +336,14 @@ __device__ __forceinline__ void store_chunk_vec(T *__restrict__ p,
 //            system in consecutive lanes: contiguous systems, solvedim 0).
 //   ORDER 1: thread t -> system slot t % spb, chunk t / spb (systems in
 //            consecutive lanes: coalesced across systems, solvedim >= 1).
-//   VEC:     16-byte vector loads/stores along the system (ORDER 0 with unit
-//            stride and 16-byte aligned chunks only; checked on the host).
+//   VEC:     1: 16-byte vector loads/stores along the system (ORDER 0 with
+//            unit stride and 16-byte aligned chunks only; checked on the
+//            host).  2: wave-staged access (ORDER 0, cps <= 64, every
+//            system exactly cps*R long and the systems of a wave adjacent in
+//            memory; checked on the host).
+//   I:       index type of element offsets (uint32_t or long long).
 // blockDim.x == cps * spb, both powers of two.
-template <typename T, int INC, int ORDER, int R, int VEC>
+template <typename T, int INC, int ORDER, int R, int VEC, typename I>
 __device__ __forceinline__ void part_block(const T *__restrict__ A,
                                            const T *__restrict__ B,
                                            const T *__restrict__ C,
@@ -258,33 +365,77 @@ __device__ __forceinline__ void part_block(const T *__restrict__ A,
   const long long q = block * spb + slot;
   const bool active = q < D.nsys;
   const int N = D.N;
-  long long off[5];
+  I off[5];
   if (active) {
-    sys_offsets(D, q, off);
+    sys_offsets<I>(D, q, off);
   } else {
 #pragma unroll
     for (int t = 0; t < 5; t++) off[t] = 0;
   }
   const int row0 = k * R;
+  const I s0 = (I)D.sstr[0], s1 = (I)D.sstr[1], s2 = (I)D.sstr[2],
+          s3 = (I)D.sstr[3], s4 = (I)D.sstr[4];
   T a[R], b[R], c[R], d[R];
   const bool full = active && (row0 + R <= N);
-  if (full) {
-    const T *pa = A + off[0] + row0 * D.sstr[0];
-    const T *pb = B + off[1] + row0 * D.sstr[1];
-    const T *pc = C + off[2] + row0 * D.sstr[2];
-    const T *pd = Dd + off[3] + row0 * D.sstr[3];
-    if (VEC) {
-      load_chunk_vec<T, R>(pa, a);
-      load_chunk_vec<T, R>(pb, b);
-      load_chunk_vec<T, R>(pc, c);
-      load_chunk_vec<T, R>(pd, d);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *stage = nullptr;
+  I wave_off[5];
+  bool wave_active = false;
+  if (VEC == 2) {
+    // the wave's span starts at the offsets of its lane 0 (chunk 0 of the
+    // wave's first system); the host guarantees that a wave's systems are
+    // either all active or all inactive
+    stage = reinterpret_cast<T *>(smem_raw) + (tid >> 6) * (kWave * R);
+    wave_active = __builtin_amdgcn_readfirstlane((int)active) != 0;
+#pragma unroll
+    for (int t = 0; t < 5; t++) {
+      if constexpr (sizeof(I) == 4) {
+        wave_off[t] = (I)__builtin_amdgcn_readfirstlane((int)off[t]);
+      } else {
+        int lo = __builtin_amdgcn_readfirstlane((int)(off[t] & 0xffffffffLL));
+        int hi = __builtin_amdgcn_readfirstlane((int)(off[t] >> 32));
+        wave_off[t] = (I)(((long long)hi << 32) | (unsigned)lo);
+      }
+    }
+  }
+  if (VEC == 2) {
+    // host guarantees: every chunk of an active wave is full and in range
+    const int lane = tid & (kWave - 1);
+    if (wave_active) {
+      load_chunk_staged<T, R>(at(A, wave_off[0]), stage, lane, a);
+      load_chunk_staged<T, R>(at(B, wave_off[1]), stage, lane, b);
+      load_chunk_staged<T, R>(at(C, wave_off[2]), stage, lane, c);
+      load_chunk_staged<T, R>(at(Dd, wave_off[3]), stage, lane, d);
+      if (row0 == 0) a[0] = T(0);
+      if (row0 + R == N) c[R - 1] = T(0);
     } else {
 #pragma unroll
       for (int r = 0; r < R; r++) {
-        a[r] = pa[r * D.sstr[0]];
-        b[r] = pb[r * D.sstr[1]];
-        c[r] = pc[r * D.sstr[2]];
-        d[r] = pd[r * D.sstr[3]];
+        a[r] = T(0);
+        b[r] = T(1);
+        c[r] = T(0);
+        d[r] = T(0);
+      }
+    }
+  } else if (full) {
+    I oa = off[0] + (I)row0 * s0, ob = off[1] + (I)row0 * s1,
+      oc = off[2] + (I)row0 * s2, od = off[3] + (I)row0 * s3;
+    if (VEC) {
+      load_chunk_vec<T, R>(at(A, oa), a);
+      load_chunk_vec<T, R>(at(B, ob), b);
+      load_chunk_vec<T, R>(at(C, oc), c);
+      load_chunk_vec<T, R>(at(Dd, od), d);
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        a[r] = ld_nt(at(A, oa));
+        b[r] = ld_nt(at(B, ob));
+        c[r] = ld_nt(at(C, oc));
+        d[r] = ld_nt(at(Dd, od));
+        oa += s0;
+        ob += s1;
+        oc += s2;
+        od += s3;
       }
     }
     if (row0 == 0) a[0] = T(0);
@@ -294,10 +445,10 @@ __device__ __forceinline__ void part_block(const T *__restrict__ A,
     for (int r = 0; r < R; r++) {
       const int row = row0 + r;
       if (active && row < N) {
-        a[r] = row == 0 ? T(0) : A[off[0] + row * D.sstr[0]];
-        b[r] = B[off[1] + row * D.sstr[1]];
-        c[r] = row == N - 1 ? T(0) : C[off[2] + row * D.sstr[2]];
-        d[r] = Dd[off[3] + row * D.sstr[3]];
+        a[r] = row == 0 ? T(0) : ld_nt(at(A, off[0] + (I)row * s0));
+        b[r] = ld_nt(at(B, off[1] + (I)row * s1));
+        c[r] = row == N - 1 ? T(0) : ld_nt(at(C, off[2] + (I)row * s2));
+        d[r] = ld_nt(at(Dd, off[3] + (I)row * s3));
       } else {
         a[r] = T(0);
         b[r] = T(1);
@@ -310,38 +461,52 @@ __device__ __forceinline__ void part_block(const T *__restrict__ A,
   Row<T> f, l;
   chunk_reduce<T, R>(a, b, c, d, f, l);
 
+  T xf, xl;
   if (ORDER == 0 && cps <= kWave) {
-    pcr_wave<T>(f, l, k, cps, tid & (kWave - 1));
+    reduced_wave<T>(f, l, k, cps, tid & (kWave - 1), xf, xl);
   } else {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    T *sm = reinterpret_cast<T *>(smem_raw);
-    const int M = 2 * cps * spb;
-    pcr_lds<T>(f, l, k, cps, slot, spb, sm, sm + M, sm + 2 * M);
+    reduced_lds<T>(f, l, k, cps, slot, spb, reinterpret_cast<T *>(smem_raw),
+                   xf, xl);
   }
 
   T y[R];
-  chunk_expand<T, R>(a, c, d, f.d, l.d, y);
+  chunk_expand<T, R>(a, c, d, xf, xl, y);
 
-  if (full) {
+  if (VEC == 2) {
+    const int lane = tid & (kWave - 1);
+    if (wave_active) {
+      if (INC)
+        store_chunk_staged<T, R>(at(U, wave_off[4]), stage, lane, y, true);
+      else
+        store_chunk_staged<T, R>(at(Dd, wave_off[3]), stage, lane, y, false);
+    }
+  } else if (full) {
     if (INC) {
-      T *pu = U + off[4] + row0 * D.sstr[4];
+      I ou = off[4] + (I)row0 * s4;
       if (VEC) {
         T u[R];
-        load_chunk_vec<T, R>(pu, u);
+        load_chunk_vec<T, R>(at(U, ou), u);
 #pragma unroll
         for (int r = 0; r < R; r++) u[r] += y[r];
-        store_chunk_vec<T, R>(pu, u);
+        store_chunk_vec<T, R>(at(U, ou), u);
       } else {
 #pragma unroll
-        for (int r = 0; r < R; r++) pu[r * D.sstr[4]] += y[r];
+        for (int r = 0; r < R; r++) {
+          T *pu = at(U, ou);
+          st_nt(pu, ld_nt(pu) + y[r]);
+          ou += s4;
+        }
       }
     } else {
-      T *pd = Dd + off[3] + row0 * D.sstr[3];
+      I od = off[3] + (I)row0 * s3;
       if (VEC) {
-        store_chunk_vec<T, R>(pd, y);
+        store_chunk_vec<T, R>(at(Dd, od), y);
       } else {
 #pragma unroll
-        for (int r = 0; r < R; r++) pd[r * D.sstr[3]] = y[r];
+        for (int r = 0; r < R; r++) {
+          st_nt(at(Dd, od), y[r]);
+          od += s3;
+        }
       }
     }
   } else if (active) {
@@ -349,10 +514,12 @@ __device__ __forceinline__ void part_block(const T *__restrict__ A,
     for (int r = 0; r < R; r++) {
       const int row = row0 + r;
       if (row < N) {
-        if (INC)
-          U[off[4] + row * D.sstr[4]] += y[r];
-        else
-          Dd[off[3] + row * D.sstr[3]] = y[r];
+        if (INC) {
+          T *pu = at(U, off[4] + (I)row * s4);
+          st_nt(pu, ld_nt(pu) + y[r]);
+        } else {
+          st_nt(at(Dd, off[3] + (I)row * s3), y[r]);
+        }
       }
     }
   }

@@ -28,7 +28,7 @@ void *workspace(size_t bytes);
 int autotune_mode();
 
 struct ForcedConfig {
-  int kind = 0, rows = 0, cps = 0, spb = 0;
+  int kind = 0, rows = 0, cps = 0, spb = 0, stage = 1;
 };
 ForcedConfig forced_config();
 

@@ -30,13 +30,13 @@ namespace tridhip {
 // ---------------------------------------------------------------------------
 // kernels
 
-template <typename T, int INC, int ORDER, int R, int VEC>
+template <typename T, int INC, int ORDER, int R, int VEC, typename I>
 __global__ void __launch_bounds__((R >= 16 && sizeof(T) == 8) ? 512 : 1024)
     k_part(const T *__restrict__ A, const T *__restrict__ B,
            const T *__restrict__ C, T *__restrict__ Dd, T *__restrict__ U,
            SysDesc D, int cps_log2, int spb_log2) {
-  part_block<T, INC, ORDER, R, VEC>(A, B, C, Dd, U, D, cps_log2, spb_log2,
-                                    blockIdx.x);
+  part_block<T, INC, ORDER, R, VEC, I>(A, B, C, Dd, U, D, cps_log2, spb_log2,
+                                       blockIdx.x);
 }
 
 // Fallback: one thread per system, plain Thomas with the forward sweep's
@@ -51,7 +51,7 @@ __global__ void __launch_bounds__(256)
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= D.nsys) return;
   long long off[5];
-  sys_offsets(D, q, off);
+  sys_offsets<long long>(D, q, off);
   const int N = D.N;
   const long long ns = D.nsys;
   T *cs = ws;
@@ -89,6 +89,13 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Applies the solver's reciprocal to n values (diagnostic entry point).
+template <typename T>
+__global__ void k_rcp(const T *__restrict__ x, T *__restrict__ y, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = rcp(x[i]);
+}
+
 // ---------------------------------------------------------------------------
 // host side
 
@@ -101,6 +108,7 @@ struct PartCfg {
   int spb_log2 = 0;
   int order = 0;
   int vec = 0;
+  int idx32 = 1;  // 32-bit element offsets (every array < 4 GiB)
 };
 
 int ilog2(long long v) {
@@ -117,10 +125,16 @@ hipError_t launch_part_t(const PartCfg &cfg, const T *a, const T *b,
   const int threads = cps * spb;
   const long long blocks = (D.nsys + spb - 1) / spb;
   size_t lds = 0;
-  if (ORDER == 1 || cps > kWave) lds = 3 * 2 * (size_t)cps * spb * sizeof(T);
-  hipLaunchKernelGGL((k_part<T, INC, ORDER, R, VEC>), dim3((unsigned)blocks),
-                     dim3(threads), lds, st, a, b, c, d, u, D, cfg.cps_log2,
-                     cfg.spb_log2);
+  if (ORDER == 1 || cps > kWave) lds = 2 * 3 * (size_t)cps * spb * sizeof(T);
+  if (VEC == 2) lds = (size_t)threads * R * sizeof(T);  // wave staging buffers
+  if (cfg.idx32)
+    hipLaunchKernelGGL((k_part<T, INC, ORDER, R, VEC, uint32_t>),
+                       dim3((unsigned)blocks), dim3(threads), lds, st, a, b, c,
+                       d, u, D, cfg.cps_log2, cfg.spb_log2);
+  else
+    hipLaunchKernelGGL((k_part<T, INC, ORDER, R, VEC, long long>),
+                       dim3((unsigned)blocks), dim3(threads), lds, st, a, b, c,
+                       d, u, D, cfg.cps_log2, cfg.spb_log2);
   return hipGetLastError();
 }
 
@@ -136,6 +150,9 @@ hipError_t launch_part(const PartCfg &cfg, const T *a, const T *b, const T *c,
   TRID_CASE(0, 4, 1)
   TRID_CASE(0, 8, 1)
   TRID_CASE(0, 16, 1)
+  TRID_CASE(0, 4, 2)
+  TRID_CASE(0, 8, 2)
+  TRID_CASE(0, 16, 2)
   TRID_CASE(1, 4, 0)
   TRID_CASE(1, 8, 0)
   TRID_CASE(1, 16, 0)
@@ -152,8 +169,6 @@ hipError_t launch_fallback(const T *a, const T *b, const T *c, T *d, T *u,
   return hipGetLastError();
 }
 
-// Largest system handled by the partitioned kernels (cps <= 1024 chunks).
-constexpr int kMaxCps = 1024;
 
 bool cfg_valid(const PartCfg &cfg, int N) {
   if (cfg.kind != 1) return true;
@@ -181,6 +196,16 @@ PartCfg make_cfg(int order, int R, int N, int vec, int threads) {
   return c;
 }
 
+// Wave-staged access (VEC 2) needs dense arrays (system q at q * N), whole
+// chunks (cps * R == N), all chunks of a system in one wave, and whole waves
+// of systems (nsys a multiple of the systems per wave).
+bool stage_ok(const PartCfg &c, bool dense, long long nsys, int N) {
+  if (c.kind != 1 || c.order != 0 || c.vec != 1 || !dense) return false;
+  const long long cps = 1LL << c.cps_log2;
+  if (cps > kWave || cps * c.R != N) return false;
+  return nsys % (kWave / cps) == 0;
+}
+
 std::string cfg_name(const PartCfg &c, size_t tsize) {
   char buf[128];
   if (c.kind == 2) {
@@ -188,24 +213,32 @@ std::string cfg_name(const PartCfg &c, size_t tsize) {
   } else {
     snprintf(buf, sizeof buf, "part<%s,o%d,R%d,cps%d,spb%d%s>",
              tsize == 8 ? "double" : "float", c.order, c.R, 1 << c.cps_log2,
-             1 << c.spb_log2, c.vec ? ",vec" : "");
+             1 << c.spb_log2, c.vec == 2 ? ",staged" : (c.vec ? ",vec" : ""));
   }
   return buf;
 }
 
-std::vector<PartCfg> candidates(int order, int N, int vec, size_t tsize) {
+std::vector<PartCfg> candidates(int order, int N, int vec, size_t tsize,
+                                bool dense, long long nsys) {
   std::vector<PartCfg> out;
   const int Rs[3] = {4, 8, 16};
   const int thr[3] = {256, 512, 1024};
+  auto add = [&](const PartCfg &c) {
+    for (auto &o : out)
+      if (o.R == c.R && o.cps_log2 == c.cps_log2 && o.spb_log2 == c.spb_log2 &&
+          o.vec == c.vec)
+        return;
+    out.push_back(c);
+  };
   for (int R : Rs)
     for (int t : thr) {
       PartCfg c = make_cfg(order, R, N, vec && (R * (int)tsize) % 16 == 0, t);
       if (!cfg_valid(c, N)) continue;
-      bool dup = false;
-      for (auto &o : out)
-        dup |= (o.R == c.R && o.cps_log2 == c.cps_log2 &&
-                o.spb_log2 == c.spb_log2 && o.vec == c.vec);
-      if (!dup) out.push_back(c);
+      add(c);
+      if (stage_ok(c, dense, nsys, N)) {
+        c.vec = 2;
+        add(c);
+      }
     }
   return out;
 }
@@ -262,10 +295,15 @@ tridStatus_t run_cfg(const PartCfg &cfg, const T *a, const T *b, const T *c,
 }
 
 template <typename T, int INC>
-PartCfg autotune(int order, int vec, const T *a, const T *b, const T *c, T *d,
-                 T *u, const SysDesc &D, long long d_extent, hipStream_t st) {
-  std::vector<PartCfg> cands = candidates(order, D.N, vec, sizeof(T));
+PartCfg autotune(int order, int vec, int idx32, bool dense, const T *a,
+                 const T *b, const T *c, T *d, T *u, const SysDesc &D,
+                 long long d_extent, hipStream_t st) {
+  std::vector<PartCfg> cands =
+      candidates(order, D.N, vec, sizeof(T), dense, D.nsys);
+  for (auto &cf : cands) cf.idx32 = idx32;
   PartCfg best = heuristic(order, D.N, vec, sizeof(T));
+  if (stage_ok(best, dense, D.nsys, D.N)) best.vec = 2;
+  best.idx32 = idx32;
   if (cands.size() < 2) return best;
   // scratch copy of the written array so that tuning leaves the caller's
   // data untouched
@@ -352,6 +390,13 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
     }
   }
   D.small_index = D.nsys < (1LL << 31);
+  // 32-bit offsets when every array (its allocated extent) fits in 4 GiB
+  int idx32 = 1;
+  for (int t = 0; t < 5; t++) {
+    if (t == 4 && !INC) break;
+    if ((unsigned long long)extent(ndim, dims, pads[t]) * sizeof(T) >= (1ULL << 32))
+      idx32 = 0;
+  }
 
   const int order = solvedim == 0 ? 0 : 1;
   // vector loads along the system need unit stride and 16-byte aligned chunks
@@ -365,6 +410,12 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
       if (D.ostr[t][k] % E != 0) vec = 0;
   }
 
+  // dense: every array is unpadded (system q of a solvedim-0 batch at q*N)
+  bool dense = true;
+  for (int t = 0; t < 5; t++) {
+    if (t == 4 && !INC) break;
+    for (int k = 0; k < ndim; k++) dense &= pads[t][k] == dims[k];
+  }
   hipStream_t st = current_stream();
   PartCfg cfg;
   ForcedConfig fc = forced_config();
@@ -378,8 +429,10 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
     if (fc.spb) cfg.spb_log2 = ilog2(fc.spb);
     if (!cfg_valid(cfg, D.N))
       return fail(TRID_STATUS_INVALID_VALUE, "forced configuration does not fit this shape");
+    if (fc.stage && stage_ok(cfg, dense, D.nsys, D.N)) cfg.vec = 2;
   } else if (autotune_mode() == 1) {
-    auto key = std::make_tuple((int)sizeof(T), INC, order, D.N, D.nsys, vec);
+    auto key = std::make_tuple((int)sizeof(T), INC, order, D.N, D.nsys,
+                               vec * 2 + (int)dense);
     bool have = false;
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -391,13 +444,16 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
     }
     if (!have) {
       long long ext = extent(ndim, dims, INC ? pads[4] : pads[3]);
-      cfg = autotune<T, INC>(order, vec, a, b, c, d, u, D, ext, st);
+      cfg = autotune<T, INC>(order, vec, idx32, dense, a, b, c, d, u, D, ext,
+                             st);
       std::lock_guard<std::mutex> lk(g_tune_mu);
       g_tuned[key] = cfg;
     }
   } else {
     cfg = heuristic(order, D.N, vec, sizeof(T));
+    if (stage_ok(cfg, dense, D.nsys, D.N)) cfg.vec = 2;
   }
+  cfg.idx32 = idx32;
   set_last_config(cfg_name(cfg, sizeof(T)));
   tridStatus_t s = run_cfg<T, INC>(cfg, a, b, c, d, u, D, st);
   if (s != TRID_STATUS_SUCCESS) return s;
@@ -409,6 +465,26 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
 }  // namespace tridhip
 
 using tridhip::solve;
+
+EXTERN_C tridStatus_t tridDebugReciprocal(const void *x, void *y, long long n,
+                                          int is_double) {
+  using namespace tridhip;
+  clear_error();
+  if (n < 0 || (n > 0 && (!x || !y)))
+    return fail(TRID_STATUS_INVALID_VALUE, "tridDebugReciprocal: bad arguments");
+  if (n == 0) return TRID_STATUS_SUCCESS;
+  hipStream_t st = current_stream();
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  if (is_double)
+    hipLaunchKernelGGL(k_rcp<double>, dim3(blocks), dim3(256), 0, st,
+                       static_cast<const double *>(x), static_cast<double *>(y), n);
+  else
+    hipLaunchKernelGGL(k_rcp<float>, dim3(blocks), dim3(256), 0, st,
+                       static_cast<const float *>(x), static_cast<float *>(y), n);
+  tridStatus_t s = check_hip(hipGetLastError(), "k_rcp launch");
+  if (s) return s;
+  return check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
 
 EXTERN_C void initTridMultiDimBatchSolve(int ndim, int *dims, int *pads) {
   (void)ndim;
