@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["adi", "plain"], default="plain")
+    ap.add_argument("--workload", choices=["adi", "plain"], default="adi")
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
@@ -99,6 +99,66 @@ def plain_workload(args, torch, np, api, ta):
                 workload=f"plain batched scalar solve {n}^3 along solvedim 0,1,2")
 
 
+def adi_workload(args, torch, np, api, ta):
+    from tridsolver_amd.adi import AdiPlan
+    n = args.n
+    dt = np.float64 if args.dtype == "f64" else np.float32
+    tdt = torch.float64 if args.dtype == "f64" else torch.float32
+    wsize = 8 if args.dtype == "f64" else 4
+    u0 = np.zeros((n, n, n), dt)
+    u0[1:-1, 1:-1, 1:-1] = np.random.default_rng(0).uniform(0, 1, (n - 2, n - 2, n - 2))
+    u = torch.from_numpy(u0).to("cuda")
+    du = torch.empty_like(u)
+    plan = AdiPlan(n, n, n, tdt, 1.0, 1)
+    pts = n ** 3
+    # algorithmic bytes per launch: rhs reads u and writes du; the x and y
+    # sweeps read and write du; the z sweep reads du and u and writes u
+    algo = [2 * wsize * pts, 2 * wsize * pts, 2 * wsize * pts, 3 * wsize * pts]
+
+    def step(events=None):
+        for g in range(4):
+            if events is not None:
+                events[g][0].record()
+            plan.stage(u, du, g)
+            if events is not None:
+                events[g][1].record()
+
+    def graph_ms(k):
+        plan.capture(u, du, k)
+        plan.replay()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        plan.replay(sync=0)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / k
+
+    return dict(step=step, kernels=("rhs", "sweep_x", "sweep_y", "sweep_z_inc"),
+                unknowns_per_step=3 * pts, algo_bytes=algo, host=[u0], dims=[n, n, n],
+                configs={g: plan.describe() for g in range(4)}, graph_ms=graph_ms,
+                workload=f"ADI heat diffusion {n}^3, lambda=1, a=c=-lambda, b=1+2lambda, "
+                         f"Dirichlet-zero, u0~U(0,1) seed 0 (apps/adi time step)")
+
+
+def cpu_baseline_adi(args, np, host, dims, seconds):
+    import oracle
+    n = dims[0]
+    u = host[0].ravel().copy()
+    t_total = 0.0
+    steps = 0
+    while t_total < seconds and steps < 20:
+        t0 = time.perf_counter()
+        assert oracle.adi(u, n, n, n, 1, 1.0, 1) == 0
+        t_total += time.perf_counter() - t0
+        steps += 1
+    return dict(value=3 * n ** 3 * steps / t_total / 1e9, unit="Gunknowns/s",
+                cores=oracle.num_threads(), kind="port",
+                sample=f"{steps} ADI time steps of the same {n}^3 workload, oracle C/OpenMP port "
+                       f"of apps/adi/src/adi_orig.c (materialised coefficients, as the reference)")
+
+
 def cpu_baseline_plain(args, np, host, dims, seconds):
     import oracle
     n = dims[0]
@@ -143,7 +203,7 @@ def main():
         return
     ta.set_stream(None)  # null stream == torch's default stream
     ta.set_autotune(args.autotune)
-    W = plain_workload(args, torch, np, api, ta)
+    W = (adi_workload if args.workload == "adi" else plain_workload)(args, torch, np, api, ta)
     for _ in range(max(args.warmup, 1)):
         W["step"]()
     torch.cuda.synchronize()
@@ -202,11 +262,13 @@ def main():
             "traffic": traffic_from_profiles(f"{W['workload']}|{W['kernels'][dom]}|{args.dtype}"),
         },
         "kernels": kernels,
+        "graph_ms_per_step": (round(W["graph_ms"](args.steps), 5) if "graph_ms" in W else None),
         "wall_s_timed_region": round(wall, 3),
         "gpu_ms_timed_region": round(gpu_ms, 3),
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline_plain(args, np, W["host"], W["dims"], args.cpu_seconds)
+        fnb = cpu_baseline_adi if args.workload == "adi" else cpu_baseline_plain
+        out["cpu_baseline"] = fnb(args, np, W["host"], W["dims"], args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

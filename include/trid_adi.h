@@ -1,0 +1,53 @@
+/*
+ * trid_adi.h -- ADI heat-diffusion time stepping on one MI355X.
+ *
+ * Replaces the time loop of the reference's ADI application
+ * (apps/adi/src/adi_cuda.cu:285-330 with the preproc kernel of
+ * apps/adi/src/preproc.cu:38-90).  The reference materialises nine
+ * coefficient arrays every step and calls tridXmtsvStridedBatch along x, y
+ * and (Inc) z; this driver keeps the constant coefficients in registers and
+ * moves only u and du: per step
+ *     du = lambda * (7-point Laplacian of u)  at interior points, 0 on the
+ *          boundary (Dirichlet rows a = c = 0, b = 1)
+ *     du <- A_x^{-1} du,  du <- A_y^{-1} du,  u <- u + A_z^{-1} du
+ * with interior rows a = c = -lambda/2, b = 1 + lambda (mode 0, the
+ * reference's coefficients, apps/adi/src/preproc.cu:70-72) or
+ * a = c = -lambda, b = 1 + 2 lambda (mode 1).
+ *
+ * Arrays are dense nx*ny*nz device arrays, x fastest; u is updated in place,
+ * du is caller-provided scratch of the same size.  All work is enqueued on
+ * the stream set by tridSetStream.  This header has no counterpart in the
+ * reference library (the reference ADI is an application, not an API).
+ */
+#ifndef TRIDSOLVER_AMD_TRID_ADI_H
+#define TRIDSOLVER_AMD_TRID_ADI_H
+
+#include "trid_common.h"
+
+typedef struct tridAdiPlan_st *tridAdiPlan_t;
+
+/* is_double: 1 = FP64, 0 = FP32.  nx, ny, nz >= 1. */
+EXTERN_C tridStatus_t tridAdiPlanCreate(tridAdiPlan_t *plan, int nx, int ny,
+                                        int nz, int is_double, double lambda,
+                                        int mode);
+EXTERN_C tridStatus_t tridAdiPlanDestroy(tridAdiPlan_t plan);
+
+/* Enqueues nsteps time steps; sync = 1 waits for them to finish. */
+EXTERN_C tridStatus_t tridAdiStep(tridAdiPlan_t plan, void *u, void *du,
+                                  int nsteps, int sync);
+
+/* Enqueues one stage of a time step: 0 = right-hand side, 1 = x sweep,
+ * 2 = y sweep, 3 = z sweep with u += (for tests and per-kernel timing). */
+EXTERN_C tridStatus_t tridAdiStage(tridAdiPlan_t plan, void *u, void *du,
+                                   int stage);
+
+/* Captures nsteps time steps on (u, du) into a HIP graph held by the plan
+ * (replacing any earlier one); tridAdiGraphLaunch replays it. */
+EXTERN_C tridStatus_t tridAdiGraphCapture(tridAdiPlan_t plan, void *u,
+                                          void *du, int nsteps);
+EXTERN_C tridStatus_t tridAdiGraphLaunch(tridAdiPlan_t plan, int sync);
+
+/* Describes the kernel configurations of the plan's four stages. */
+EXTERN_C const char *tridAdiPlanDescribe(tridAdiPlan_t plan);
+
+#endif

@@ -30,6 +30,14 @@ SIGNATURES = {
     "tridGetLastConfig": (C.c_char_p, []),
     "tridForceConfig": (ST, [I, I, I, I]),
     "tridDebugReciprocal": (ST, [P, P, C.c_longlong, I]),
+    # trid_adi.h
+    "tridAdiPlanCreate": (ST, [C.POINTER(P), I, I, I, I, C.c_double, I]),
+    "tridAdiPlanDestroy": (ST, [P]),
+    "tridAdiStep": (ST, [P, P, P, I, I]),
+    "tridAdiStage": (ST, [P, P, P, I]),
+    "tridAdiGraphCapture": (ST, [P, P, P, I]),
+    "tridAdiGraphLaunch": (ST, [P, I]),
+    "tridAdiPlanDescribe": (C.c_char_p, [P]),
 }
 
 

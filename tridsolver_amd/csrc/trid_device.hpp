@@ -26,17 +26,21 @@ struct SysDesc {
 
 // Element offsets of system q in the five arrays, in index type I (uint32_t
 // when the host has checked that every array's extent fits, else int64).
+// `interior` is set when every non-solve coordinate of the system lies in
+// [1, size-2] (the system is a line through the interior of the grid).
 template <typename I>
 __device__ __forceinline__ void sys_offsets(const SysDesc &D, long long q,
-                                            I off[5]) {
+                                            I off[5], bool *interior = nullptr) {
 #pragma unroll
   for (int t = 0; t < 5; t++) off[t] = 0;
+  bool in = true;
   if (D.small_index) {
     uint32_t qq = (uint32_t)q;
     for (int k = 0; k < D.nother; k++) {
       uint32_t sz = (uint32_t)D.osize[k];
       uint32_t ik = qq % sz;
       qq /= sz;
+      in &= ik >= 1 && ik + 2 <= sz;
 #pragma unroll
       for (int t = 0; t < 5; t++) off[t] += (I)ik * (I)D.ostr[t][k];
     }
@@ -45,10 +49,12 @@ __device__ __forceinline__ void sys_offsets(const SysDesc &D, long long q,
       long long sz = D.osize[k];
       long long ik = q % sz;
       q /= sz;
+      in &= ik >= 1 && ik + 2 <= sz;
 #pragma unroll
       for (int t = 0; t < 5; t++) off[t] += (I)ik * (I)D.ostr[t][k];
     }
   }
+  if (interior) *interior = in;
 }
 
 // Reciprocal.  Default: the hardware estimate (v_rcp_f64 / v_rcp_f32)

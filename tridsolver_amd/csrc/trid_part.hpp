@@ -331,6 +331,26 @@ __device__ __forceinline__ void store_chunk_staged(T *__restrict__ span,
   __builtin_amdgcn_wave_barrier();
 }
 
+// Coefficient sources of part_block.
+//   CoefArrays: a, b, c read from memory with the layout of SysDesc (arrays
+//               0, 1, 2) -- the library's strided-batch entry points.
+//   CoefConst:  constant a, b, c on rows interior to the grid (the system's
+//               line is interior and 1 <= row <= N-2), identity rows
+//               (a = c = 0, b = 1) elsewhere -- the ADI sweeps with
+//               Dirichlet boundaries.
+template <typename T>
+struct CoefArrays {
+  static constexpr bool kArrays = true;
+  const T *__restrict__ A;
+  const T *__restrict__ B;
+  const T *__restrict__ C;
+};
+template <typename T>
+struct CoefConst {
+  static constexpr bool kArrays = false;
+  T a, b, c;
+};
+
 // Block-wide partitioned solve of a strided batch.
 //   ORDER 0: thread t -> chunk t % cps, system slot t / cps (chunks of a
 //            system in consecutive lanes: contiguous systems, solvedim 0).
@@ -342,12 +362,11 @@ __device__ __forceinline__ void store_chunk_staged(T *__restrict__ span,
 //            system exactly cps*R long and the systems of a wave adjacent in
 //            memory; checked on the host).
 //   I:       index type of element offsets (uint32_t or long long).
-// blockDim.x == cps * spb, both powers of two.
-template <typename T, int INC, int ORDER, int R, int VEC, typename I>
-__device__ __forceinline__ void part_block(const T *__restrict__ A,
-                                           const T *__restrict__ B,
-                                           const T *__restrict__ C,
-                                           T *__restrict__ Dd,
+// blockDim.x == cps * spb, both powers of two.  Non-Inc: d <- A^{-1} d;
+// Inc: u <- u + A^{-1} d.
+template <typename T, int INC, int ORDER, int R, int VEC, typename I,
+          typename Coef>
+__device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
                                            T *__restrict__ U, const SysDesc &D,
                                            int cps_log2, int spb_log2,
                                            long long block) {
@@ -366,8 +385,9 @@ __device__ __forceinline__ void part_block(const T *__restrict__ A,
   const bool active = q < D.nsys;
   const int N = D.N;
   I off[5];
+  bool line_in = false;
   if (active) {
-    sys_offsets<I>(D, q, off);
+    sys_offsets<I>(D, q, off, &line_in);
   } else {
 #pragma unroll
     for (int t = 0; t < 5; t++) off[t] = 0;
@@ -377,6 +397,7 @@ __device__ __forceinline__ void part_block(const T *__restrict__ A,
           s3 = (I)D.sstr[3], s4 = (I)D.sstr[4];
   T a[R], b[R], c[R], d[R];
   const bool full = active && (row0 + R <= N);
+
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *stage = nullptr;
   I wave_off[5];
@@ -398,16 +419,28 @@ __device__ __forceinline__ void part_block(const T *__restrict__ A,
       }
     }
   }
+
+  // ---- load ------------------------------------------------------------
+  if constexpr (!Coef::kArrays) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int row = row0 + r;
+      const bool in = line_in && row >= 1 && row + 2 <= N;
+      a[r] = in ? coef.a : T(0);
+      b[r] = in ? coef.b : T(1);
+      c[r] = in ? coef.c : T(0);
+    }
+  }
   if (VEC == 2) {
     // host guarantees: every chunk of an active wave is full and in range
     const int lane = tid & (kWave - 1);
     if (wave_active) {
-      load_chunk_staged<T, R>(at(A, wave_off[0]), stage, lane, a);
-      load_chunk_staged<T, R>(at(B, wave_off[1]), stage, lane, b);
-      load_chunk_staged<T, R>(at(C, wave_off[2]), stage, lane, c);
+      if constexpr (Coef::kArrays) {
+        load_chunk_staged<T, R>(at(coef.A, wave_off[0]), stage, lane, a);
+        load_chunk_staged<T, R>(at(coef.B, wave_off[1]), stage, lane, b);
+        load_chunk_staged<T, R>(at(coef.C, wave_off[2]), stage, lane, c);
+      }
       load_chunk_staged<T, R>(at(Dd, wave_off[3]), stage, lane, d);
-      if (row0 == 0) a[0] = T(0);
-      if (row0 + R == N) c[R - 1] = T(0);
     } else {
 #pragma unroll
       for (int r = 0; r < R; r++) {
@@ -421,16 +454,20 @@ __device__ __forceinline__ void part_block(const T *__restrict__ A,
     I oa = off[0] + (I)row0 * s0, ob = off[1] + (I)row0 * s1,
       oc = off[2] + (I)row0 * s2, od = off[3] + (I)row0 * s3;
     if (VEC) {
-      load_chunk_vec<T, R>(at(A, oa), a);
-      load_chunk_vec<T, R>(at(B, ob), b);
-      load_chunk_vec<T, R>(at(C, oc), c);
+      if constexpr (Coef::kArrays) {
+        load_chunk_vec<T, R>(at(coef.A, oa), a);
+        load_chunk_vec<T, R>(at(coef.B, ob), b);
+        load_chunk_vec<T, R>(at(coef.C, oc), c);
+      }
       load_chunk_vec<T, R>(at(Dd, od), d);
     } else {
 #pragma unroll
       for (int r = 0; r < R; r++) {
-        a[r] = ld_nt(at(A, oa));
-        b[r] = ld_nt(at(B, ob));
-        c[r] = ld_nt(at(C, oc));
+        if constexpr (Coef::kArrays) {
+          a[r] = ld_nt(at(coef.A, oa));
+          b[r] = ld_nt(at(coef.B, ob));
+          c[r] = ld_nt(at(coef.C, oc));
+        }
         d[r] = ld_nt(at(Dd, od));
         oa += s0;
         ob += s1;
@@ -438,16 +475,16 @@ __device__ __forceinline__ void part_block(const T *__restrict__ A,
         od += s3;
       }
     }
-    if (row0 == 0) a[0] = T(0);
-    if (row0 + R == N) c[R - 1] = T(0);
   } else {
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int row = row0 + r;
       if (active && row < N) {
-        a[r] = row == 0 ? T(0) : ld_nt(at(A, off[0] + (I)row * s0));
-        b[r] = ld_nt(at(B, off[1] + (I)row * s1));
-        c[r] = row == N - 1 ? T(0) : ld_nt(at(C, off[2] + (I)row * s2));
+        if constexpr (Coef::kArrays) {
+          a[r] = row == 0 ? T(0) : ld_nt(at(coef.A, off[0] + (I)row * s0));
+          b[r] = ld_nt(at(coef.B, off[1] + (I)row * s1));
+          c[r] = row == N - 1 ? T(0) : ld_nt(at(coef.C, off[2] + (I)row * s2));
+        }
         d[r] = ld_nt(at(Dd, off[3] + (I)row * s3));
       } else {
         a[r] = T(0);
@@ -457,7 +494,16 @@ __device__ __forceinline__ void part_block(const T *__restrict__ A,
       }
     }
   }
+  if constexpr (Coef::kArrays) {
+    // a_0 and c_{N-1} of every system are never used (the partial-chunk
+    // path above already masked them)
+    if (VEC == 2 || full) {
+      if (row0 == 0) a[0] = T(0);
+      if (row0 + R == N) c[R - 1] = T(0);
+    }
+  }
 
+  // ---- solve -----------------------------------------------------------
   Row<T> f, l;
   chunk_reduce<T, R>(a, b, c, d, f, l);
 
@@ -472,6 +518,7 @@ __device__ __forceinline__ void part_block(const T *__restrict__ A,
   T y[R];
   chunk_expand<T, R>(a, c, d, xf, xl, y);
 
+  // ---- store -----------------------------------------------------------
   if (VEC == 2) {
     const int lane = tid & (kWave - 1);
     if (wave_active) {

@@ -22,22 +22,13 @@
 
 #include "../../include/trid_hip.h"
 #include "../../include/trid_hip_ext.h"
-#include "trid_part.hpp"
+#include "trid_launch.hpp"
 #include "trid_runtime.hpp"
 
 namespace tridhip {
 
 // ---------------------------------------------------------------------------
 // kernels
-
-template <typename T, int INC, int ORDER, int R, int VEC, typename I>
-__global__ void __launch_bounds__((R >= 16 && sizeof(T) == 8) ? 512 : 1024)
-    k_part(const T *__restrict__ A, const T *__restrict__ B,
-           const T *__restrict__ C, T *__restrict__ Dd, T *__restrict__ U,
-           SysDesc D, int cps_log2, int spb_log2) {
-  part_block<T, INC, ORDER, R, VEC, I>(A, B, C, Dd, U, D, cps_log2, spb_log2,
-                                       blockIdx.x);
-}
 
 // Fallback: one thread per system, plain Thomas with the forward sweep's
 // c' and d' in a global workspace laid out [row][system] (coalesced across
@@ -101,65 +92,6 @@ __global__ void k_rcp(const T *__restrict__ x, T *__restrict__ y, long long n) {
 
 namespace {
 
-struct PartCfg {
-  int kind = 1;  // 1 partitioned, 2 fallback
-  int R = 8;
-  int cps_log2 = 0;
-  int spb_log2 = 0;
-  int order = 0;
-  int vec = 0;
-  int idx32 = 1;  // 32-bit element offsets (every array < 4 GiB)
-};
-
-int ilog2(long long v) {
-  int l = 0;
-  while ((1LL << l) < v) l++;
-  return l;
-}
-
-template <typename T, int INC, int ORDER, int R, int VEC>
-hipError_t launch_part_t(const PartCfg &cfg, const T *a, const T *b,
-                         const T *c, T *d, T *u, const SysDesc &D,
-                         hipStream_t st) {
-  const int cps = 1 << cfg.cps_log2, spb = 1 << cfg.spb_log2;
-  const int threads = cps * spb;
-  const long long blocks = (D.nsys + spb - 1) / spb;
-  size_t lds = 0;
-  if (ORDER == 1 || cps > kWave) lds = 2 * 3 * (size_t)cps * spb * sizeof(T);
-  if (VEC == 2) lds = (size_t)threads * R * sizeof(T);  // wave staging buffers
-  if (cfg.idx32)
-    hipLaunchKernelGGL((k_part<T, INC, ORDER, R, VEC, uint32_t>),
-                       dim3((unsigned)blocks), dim3(threads), lds, st, a, b, c,
-                       d, u, D, cfg.cps_log2, cfg.spb_log2);
-  else
-    hipLaunchKernelGGL((k_part<T, INC, ORDER, R, VEC, long long>),
-                       dim3((unsigned)blocks), dim3(threads), lds, st, a, b, c,
-                       d, u, D, cfg.cps_log2, cfg.spb_log2);
-  return hipGetLastError();
-}
-
-template <typename T, int INC>
-hipError_t launch_part(const PartCfg &cfg, const T *a, const T *b, const T *c,
-                       T *d, T *u, const SysDesc &D, hipStream_t st) {
-#define TRID_CASE(ORD, RR, VV)                                                 \
-  if (cfg.order == ORD && cfg.R == RR && cfg.vec == VV)                        \
-    return launch_part_t<T, INC, ORD, RR, VV>(cfg, a, b, c, d, u, D, st);
-  TRID_CASE(0, 4, 0)
-  TRID_CASE(0, 8, 0)
-  TRID_CASE(0, 16, 0)
-  TRID_CASE(0, 4, 1)
-  TRID_CASE(0, 8, 1)
-  TRID_CASE(0, 16, 1)
-  TRID_CASE(0, 4, 2)
-  TRID_CASE(0, 8, 2)
-  TRID_CASE(0, 16, 2)
-  TRID_CASE(1, 4, 0)
-  TRID_CASE(1, 8, 0)
-  TRID_CASE(1, 16, 0)
-#undef TRID_CASE
-  return hipErrorInvalidValue;
-}
-
 template <typename T, int INC>
 hipError_t launch_fallback(const T *a, const T *b, const T *c, T *d, T *u,
                            const SysDesc &D, hipStream_t st, T *ws) {
@@ -169,101 +101,6 @@ hipError_t launch_fallback(const T *a, const T *b, const T *c, T *d, T *u,
   return hipGetLastError();
 }
 
-
-bool cfg_valid(const PartCfg &cfg, int N) {
-  if (cfg.kind != 1) return true;
-  const long long cps = 1LL << cfg.cps_log2, spb = 1LL << cfg.spb_log2;
-  const long long thr = cps * spb;
-  if (thr < kWave || thr > 1024) return false;
-  if (cfg.R >= 16 && thr > 512) return false;  // register budget of R=16
-  if (cps * cfg.R < N) return false;
-  return true;
-}
-
-// Threads per block preferred for a given chunk count.
-PartCfg make_cfg(int order, int R, int N, int vec, int threads) {
-  PartCfg c;
-  c.kind = 1;
-  c.order = order;
-  c.R = R;
-  c.vec = vec;
-  long long chunks = (std::max(N, 1) + R - 1) / R;
-  c.cps_log2 = ilog2(chunks);
-  int cps = 1 << c.cps_log2;
-  int spb = std::max(1, threads / cps);
-  if (order == 1) spb = std::min(spb, 64);
-  c.spb_log2 = ilog2(spb);
-  return c;
-}
-
-// Wave-staged access (VEC 2) needs dense arrays (system q at q * N), whole
-// chunks (cps * R == N), all chunks of a system in one wave, and whole waves
-// of systems (nsys a multiple of the systems per wave).
-bool stage_ok(const PartCfg &c, bool dense, long long nsys, int N) {
-  if (c.kind != 1 || c.order != 0 || c.vec != 1 || !dense) return false;
-  const long long cps = 1LL << c.cps_log2;
-  if (cps > kWave || cps * c.R != N) return false;
-  return nsys % (kWave / cps) == 0;
-}
-
-std::string cfg_name(const PartCfg &c, size_t tsize) {
-  char buf[128];
-  if (c.kind == 2) {
-    snprintf(buf, sizeof buf, "fallback<%s>", tsize == 8 ? "double" : "float");
-  } else {
-    snprintf(buf, sizeof buf, "part<%s,o%d,R%d,cps%d,spb%d%s>",
-             tsize == 8 ? "double" : "float", c.order, c.R, 1 << c.cps_log2,
-             1 << c.spb_log2, c.vec == 2 ? ",staged" : (c.vec ? ",vec" : ""));
-  }
-  return buf;
-}
-
-std::vector<PartCfg> candidates(int order, int N, int vec, size_t tsize,
-                                bool dense, long long nsys) {
-  std::vector<PartCfg> out;
-  const int Rs[3] = {4, 8, 16};
-  const int thr[3] = {256, 512, 1024};
-  auto add = [&](const PartCfg &c) {
-    for (auto &o : out)
-      if (o.R == c.R && o.cps_log2 == c.cps_log2 && o.spb_log2 == c.spb_log2 &&
-          o.vec == c.vec)
-        return;
-    out.push_back(c);
-  };
-  for (int R : Rs)
-    for (int t : thr) {
-      PartCfg c = make_cfg(order, R, N, vec && (R * (int)tsize) % 16 == 0, t);
-      if (!cfg_valid(c, N)) continue;
-      add(c);
-      if (stage_ok(c, dense, nsys, N)) {
-        c.vec = 2;
-        add(c);
-      }
-    }
-  return out;
-}
-
-PartCfg heuristic(int order, int N, int vec, size_t tsize) {
-  // preferred chunk lengths: FP64 8 rows, FP32 16 rows (register budget)
-  const int pref[3] = {tsize == 8 ? 8 : 16, tsize == 8 ? 16 : 8, 4};
-  for (int R : pref) {
-    int v = vec && (R * (int)tsize) % 16 == 0;
-    PartCfg c = make_cfg(order, R, N, v, order == 0 ? 256 : 512);
-    if (cfg_valid(c, N)) return c;
-    c = make_cfg(order, R, N, v, 1024);
-    if (cfg_valid(c, N)) return c;
-  }
-  for (int R : pref) {
-    for (int t : {64, 128, 256, 512, 1024}) {
-      int v = vec && (R * (int)tsize) % 16 == 0;
-      PartCfg c = make_cfg(order, R, N, v, t);
-      if (cfg_valid(c, N)) return c;
-    }
-  }
-  PartCfg f;
-  f.kind = 2;
-  return f;
-}
 
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int, int, long long, int>, PartCfg> g_tuned;
@@ -289,7 +126,7 @@ tridStatus_t run_cfg(const PartCfg &cfg, const T *a, const T *b, const T *c,
     if (!ws) return fail(TRID_STATUS_ALLOC_FAILED, "workspace allocation failed");
     e = launch_fallback<T, INC>(a, b, c, d, u, D, st, ws);
   } else {
-    e = launch_part<T, INC>(cfg, a, b, c, d, u, D, st);
+    e = launch_part<T, INC>(cfg, CoefArrays<T>{a, b, c}, d, u, D, st);
   }
   return check_hip(e, "kernel launch");
 }
@@ -323,10 +160,12 @@ PartCfg autotune(int order, int vec, int idx32, bool dense, const T *a,
       break;
     T *dd = INC ? d : scratch;
     T *uu = INC ? scratch : u;
-    if (launch_part<T, INC>(cf, a, b, c, dd, uu, D, st) != hipSuccess) continue;
+    if (launch_part<T, INC>(cf, CoefArrays<T>{a, b, c}, dd, uu, D, st) != hipSuccess)
+      continue;
     (void)hipEventRecord(e0, st);
     const int reps = 3;
-    for (int r = 0; r < reps; r++) (void)launch_part<T, INC>(cf, a, b, c, dd, uu, D, st);
+    for (int r = 0; r < reps; r++)
+      (void)launch_part<T, INC>(cf, CoefArrays<T>{a, b, c}, dd, uu, D, st);
     (void)hipEventRecord(e1, st);
     if (hipEventSynchronize(e1) != hipSuccess) break;
     float ms = 0.f;
