@@ -111,17 +111,26 @@ def adi_workload(args, torch, np, api, ta):
     du = torch.empty_like(u)
     plan = AdiPlan(n, n, n, tdt, 1.0, 1)
     pts = n ** 3
-    # algorithmic bytes per launch: rhs reads u and writes du; the x and y
-    # sweeps read and write du; the z sweep reads du and u and writes u
-    algo = [2 * wsize * pts, 2 * wsize * pts, 2 * wsize * pts, 3 * wsize * pts]
+    fused = "fused" in plan.describe()
+    # algorithmic bytes per launch: rhs (+x when fused) reads u and writes du;
+    # the x and y sweeps read and write du; the z sweep reads du and u and
+    # writes u
+    if fused:
+        stages = (4, 2, 3)
+        names = ("rhs_x_fused", "sweep_y", "sweep_z_inc")
+        algo = [2 * wsize * pts, 2 * wsize * pts, 3 * wsize * pts]
+    else:
+        stages = (0, 1, 2, 3)
+        names = ("rhs", "sweep_x", "sweep_y", "sweep_z_inc")
+        algo = [2 * wsize * pts, 2 * wsize * pts, 2 * wsize * pts, 3 * wsize * pts]
 
     def step(events=None):
-        for g in range(4):
+        for i, g in enumerate(stages):
             if events is not None:
-                events[g][0].record()
+                events[i][0].record()
             plan.stage(u, du, g)
             if events is not None:
-                events[g][1].record()
+                events[i][1].record()
 
     def graph_ms(k):
         plan.capture(u, du, k)
@@ -135,9 +144,9 @@ def adi_workload(args, torch, np, api, ta):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / k
 
-    return dict(step=step, kernels=("rhs", "sweep_x", "sweep_y", "sweep_z_inc"),
+    return dict(step=step, kernels=names,
                 unknowns_per_step=3 * pts, algo_bytes=algo, host=[u0], dims=[n, n, n],
-                configs={g: plan.describe() for g in range(4)}, graph_ms=graph_ms,
+                configs={i: plan.describe() for i in range(len(names))}, graph_ms=graph_ms,
                 workload=f"ADI heat diffusion {n}^3, lambda=1, a=c=-lambda, b=1+2lambda, "
                          f"Dirichlet-zero, u0~U(0,1) seed 0 (apps/adi time step)")
 

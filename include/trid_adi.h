@@ -37,7 +37,10 @@ EXTERN_C tridStatus_t tridAdiStep(tridAdiPlan_t plan, void *u, void *du,
                                   int nsteps, int sync);
 
 /* Enqueues one stage of a time step: 0 = right-hand side, 1 = x sweep,
- * 2 = y sweep, 3 = z sweep with u += (for tests and per-kernel timing). */
+ * 2 = y sweep, 3 = z sweep with u += , 4 = right-hand side fused with the x
+ * sweep (for tests and per-kernel timing).  tridAdiStep uses 4, 2, 3 when
+ * the plan supports the fused stage (tridAdiPlanDescribe says so) and u, du
+ * are 16-byte aligned, else 0, 1, 2, 3. */
 EXTERN_C tridStatus_t tridAdiStage(tridAdiPlan_t plan, void *u, void *du,
                                    int stage);
 

@@ -148,6 +148,31 @@ def test_adi_stages_vs_oracle():
     assert oracle.rel_err(uexp, u.cpu().numpy().ravel()) < 1e-13
 
 
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("shape", [(64, 40, 24), (256, 16, 12), (128, 6, 5)])
+def test_adi_fused_rhs_x_equals_separate_stages(dt, shape):
+    """The fused RHS + x sweep equals the RHS kernel followed by the x sweep
+    (same operations and chunking; the compiler may fuse different
+    multiply-adds in the two kernels, so equality is to a few ulp).  Every
+    element is written (the output starts as NaN)."""
+    import torch
+    nx, ny, nz = shape
+    u0 = rand_init(nx, ny, nz, dt, seed=11)
+    plan = AdiPlan(nx, ny, nz, torch_dt(dt), 1.0, 1)
+    if "fused" not in plan.describe():
+        pytest.skip("no fused stage for this shape: " + plan.describe())
+    u = torch.from_numpy(u0.copy()).to("cuda")
+    d1 = torch.empty_like(u)
+    d2 = torch.full_like(u, float("nan"))
+    plan.stage(u, d1, 0)
+    plan.stage(u, d1, 1)
+    plan.stage(u, d2, 4)
+    a, b = d1.cpu().numpy(), d2.cpu().numpy()
+    assert not np.isnan(b).any()
+    err = oracle.rel_err(a, b)
+    assert err <= (1e-14 if dt == np.float64 else 1e-6), f"{err:.2e} [{plan.describe()}]"
+
+
 def test_adi_graph_replays_match_eager():
     import torch
     nx, ny, nz = 64, 48, 40

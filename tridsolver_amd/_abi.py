@@ -38,6 +38,14 @@ SIGNATURES = {
     "tridAdiGraphCapture": (ST, [P, P, P, I]),
     "tridAdiGraphLaunch": (ST, [P, I]),
     "tridAdiPlanDescribe": (C.c_char_p, [P]),
+    # blktrid_hip.h
+    "sbtsv_gpu": (None, [P, P, P, P, P, P, P, I, I, I]),
+    "dbtsv_gpu": (None, [P, P, P, P, P, P, P, I, I, I]),
+    "tridSbtsvStridedBatch": (ST, [P, P, P, P, P, P, P, I, I, I, I]),
+    "tridDbtsvStridedBatch": (ST, [P, P, P, P, P, P, P, I, I, I, I]),
+    "tridDbtsvMixedBatch": (ST, [P, I, I]),
+    "tridDbtsvAdjoint": (ST, [P, P, P, P, P, P, P, P, P, I, I, I, I]),
+    "tridBlkForceLanes": (ST, [I]),
 }
 
 

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/trid_adi.h"
 #include "../../include/trid_hip_ext.h"
@@ -66,6 +67,117 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Wave-staged load of 64*R contiguous elements with ordinary (cached) loads:
+// u is read by five kernels' worth of neighbours, so it stays cacheable.
+template <typename T, int R>
+__device__ __forceinline__ void load_span_cached(const T *__restrict__ span,
+                                                 T *__restrict__ buf, int lane,
+                                                 T (&x)[R], T *left = nullptr,
+                                                 T *right = nullptr) {
+  using V = typename Vec16<T>::type;
+  constexpr int E = Vec16<T>::n;
+  const V *g = reinterpret_cast<const V *>(span);
+  V *s = reinterpret_cast<V *>(buf);
+  V t[R / E];
+#pragma unroll
+  for (int v = 0; v < R / E; v++) t[v] = g[v * kWave + lane];
+#pragma unroll
+  for (int v = 0; v < R / E; v++) s[v * kWave + lane] = t[v];
+  __builtin_amdgcn_wave_barrier();
+  const V *mine = reinterpret_cast<const V *>(buf + lane * R);
+#pragma unroll
+  for (int v = 0; v < R / E; v++) {
+    V w = mine[v];
+#pragma unroll
+    for (int e = 0; e < E; e++) x[E * v + e] = w[e];
+  }
+  if (left) *left = lane > 0 ? buf[lane * R - 1] : T(0);
+  if (right) *right = lane < kWave - 1 ? buf[lane * R + R] : T(0);
+  __builtin_amdgcn_wave_barrier();
+}
+
+// The 7-point sum in the reference's order (apps/adi/src/adi_orig.c:183-186),
+// without contraction.
+template <typename T>
+__device__ __forceinline__ T rhs_point(T l, T r, T ym, T yp, T zm, T zp, T c,
+                                       T lam) {
+#pragma clang fp contract(off)
+  T s = l + r;
+  s = s + ym;
+  s = s + yp;
+  s = s + zm;
+  s = s + zp;
+  const T six = T(6) * c;
+  s = s - six;
+  return lam * s;
+}
+
+// Fused right-hand side + x sweep: du <- A_x^{-1} (lambda * Laplacian u).
+// One wave owns 64/cps whole x-lines of one z-plane (the host requires
+// nx == cps*R, cps <= 64 and ny a multiple of the lines per wave, so a wave
+// never straddles planes).  The wave's lines and their four neighbour lines
+// (y-1, y+1, z-1, z+1: contiguous spans at -nx, +nx, -nx*ny, +nx*ny) are
+// loaded as whole spans; the x-neighbours come from the staging buffer.
+// HBM traffic: u once (neighbour spans hit L2 / MALL) and du once.
+template <typename T, int R>
+__global__ void __launch_bounds__(256)
+    k_adi_rhs_x(const T *__restrict__ u, T *__restrict__ du, int nx, int ny,
+                int nz, T lam, CoefConst<T> cf, int cps_log2) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1);
+  const int cps = 1 << cps_log2;
+  const int k = tid & (cps - 1);
+  const int lines_per_block = 256 >> cps_log2;
+  const long long nlines = (long long)ny * nz;
+  const long long line = (long long)blockIdx.x * lines_per_block + (tid >> cps_log2);
+  const long long wline0 =
+      (long long)blockIdx.x * lines_per_block + ((tid & ~(kWave - 1)) >> cps_log2);
+  const bool wave_active = wline0 < nlines;
+  const long long nxy = (long long)nx * ny;
+  const int kw = (int)(wline0 / ny);
+  const bool plane_in = kw >= 1 && kw <= nz - 2;
+  const int jl = (int)(line % ny);
+  const bool line_in = plane_in && jl >= 1 && jl <= ny - 2;
+  T *stage = reinterpret_cast<T *>(smem_raw) + (tid >> 6) * (kWave * R);
+  const long long wo = wline0 * nx;
+  const int row0 = k * R;
+
+  T a[R], b[R], c[R], d[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int row = row0 + r;
+    const bool in = line_in && row >= 1 && row + 2 <= nx;
+    a[r] = in ? cf.a : T(0);
+    b[r] = in ? cf.b : T(1);
+    c[r] = in ? cf.c : T(0);
+    d[r] = T(0);
+  }
+  if (wave_active && plane_in) {
+    T uc[R], ul, ur;
+    load_span_cached<T, R>(u + wo, stage, lane, uc, &ul, &ur);
+    T ym[R], yp[R], zm[R], zp[R];
+    load_span_cached<T, R>(u + wo - nx, stage, lane, ym);
+    load_span_cached<T, R>(u + wo + nx, stage, lane, yp);
+    load_span_cached<T, R>(u + wo - nxy, stage, lane, zm);
+    load_span_cached<T, R>(u + wo + nxy, stage, lane, zp);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int row = row0 + r;
+      const bool in = line_in && row >= 1 && row + 2 <= nx;
+      const T l = r == 0 ? ul : uc[r - 1];
+      const T rr = r == R - 1 ? ur : uc[r + 1];
+      d[r] = in ? rhs_point(l, rr, ym[r], yp[r], zm[r], zp[r], uc[r], lam) : T(0);
+    }
+  }
+  Row<T> f, l;
+  chunk_reduce<T, R>(a, b, c, d, f, l);
+  T xf, xl;
+  reduced_wave<T>(f, l, k, cps, lane, xf, xl);
+  T y[R];
+  chunk_expand<T, R>(a, c, d, xf, xl, y);
+  if (wave_active) store_chunk_staged<T, R>(du + wo, stage, lane, y, false);
+}
+
 namespace {
 
 SysDesc grid_desc(int nx, int ny, int nz, int dir) {
@@ -100,6 +212,8 @@ struct tridAdiPlan_st {
   SysDesc desc[3];
   PartCfg cfg[3];      // x (16-byte aligned arrays), y, z
   PartCfg cfg_x_unaligned;
+  int fused_R = 0;          // >0: fused RHS + x sweep with this chunk length
+  int fused_cps_log2 = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t cap = nullptr;
@@ -108,6 +222,8 @@ struct tridAdiPlan_st {
 
 namespace {
 
+bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
 template <typename T>
 CoefConst<T> coefs(const tridAdiPlan_st *p) {
   const T lam = (T)p->lambda;
@@ -115,13 +231,35 @@ CoefConst<T> coefs(const tridAdiPlan_st *p) {
   return CoefConst<T>{-lam, (T)1.0f + (T)2.0f * lam, -lam};
 }
 
-bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+template <typename T>
+hipError_t launch_fused(tridAdiPlan_st *p, T *u, T *du, hipStream_t st) {
+  const CoefConst<T> cf = coefs<T>(p);
+  const int lines_per_block = 256 >> p->fused_cps_log2;
+  const long long nlines = (long long)p->ny * p->nz;
+  const unsigned blocks = (unsigned)((nlines + lines_per_block - 1) / lines_per_block);
+  const size_t lds = 256 * (size_t)p->fused_R * sizeof(T);
+  if (p->fused_R == 8)
+    hipLaunchKernelGGL((k_adi_rhs_x<T, 8>), dim3(blocks), dim3(256), lds, st, u, du,
+                       p->nx, p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2);
+  else if (p->fused_R == 4)
+    hipLaunchKernelGGL((k_adi_rhs_x<T, 4>), dim3(blocks), dim3(256), lds, st, u, du,
+                       p->nx, p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2);
+  else
+    hipLaunchKernelGGL((k_adi_rhs_x<T, 16>), dim3(blocks), dim3(256), lds, st, u, du,
+                       p->nx, p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2);
+  return hipGetLastError();
+}
 
+bool aligned16(const void *p);
+
+// Stage 4 = fused RHS + x sweep (plans with fused_R > 0 only).
 template <typename T>
 tridStatus_t stage_t(tridAdiPlan_st *p, T *u, T *du, int stage, hipStream_t st) {
   hipError_t e = hipSuccess;
   const CoefConst<T> cf = coefs<T>(p);
-  if (stage == 0) {
+  if (stage == 4) {
+    e = launch_fused<T>(p, u, du, st);
+  } else if (stage == 0) {
     dim3 grid((p->nx + 63) / 64, (p->ny + 3) / 4, (p->nz + kRhsPlanes - 1) / kRhsPlanes);
     hipLaunchKernelGGL(k_adi_rhs<T>, grid, dim3(64, 4), 0, st, u, du, p->nx, p->ny,
                        p->nz, (T)p->lambda);
@@ -142,13 +280,87 @@ tridStatus_t stage_any(tridAdiPlan_st *p, void *u, void *du, int stage, hipStrea
   return stage_t<float>(p, (float *)u, (float *)du, stage, st);
 }
 
+bool use_fused(const tridAdiPlan_st *p, const void *u, const void *du) {
+  return p->fused_R > 0 && aligned16(u) && aligned16(du);
+}
+
 tridStatus_t steps_any(tridAdiPlan_st *p, void *u, void *du, int nsteps, hipStream_t st) {
-  for (int s = 0; s < nsteps; s++)
-    for (int g = 0; g < 4; g++) {
-      tridStatus_t r = stage_any(p, u, du, g, st);
-      if (r) return r;
+  const bool fused = use_fused(p, u, du);
+  for (int s = 0; s < nsteps; s++) {
+    if (fused) {
+      for (int g : {4, 2, 3}) {
+        tridStatus_t r = stage_any(p, u, du, g, st);
+        if (r) return r;
+      }
+    } else {
+      for (int g = 0; g < 4; g++) {
+        tridStatus_t r = stage_any(p, u, du, g, st);
+        if (r) return r;
+      }
     }
+  }
   return TRID_STATUS_SUCCESS;
+}
+
+// Times every candidate configuration of the y and z sweeps (and of the x
+// sweep) on scratch grids and keeps the fastest.  Data values do not change
+// the kernels' work, so zero-filled scratch is enough.
+template <typename T>
+void tune_plan_t(tridAdiPlan_st *p) {
+  const size_t bytes = (size_t)p->nx * p->ny * p->nz * sizeof(T);
+  T *u = nullptr, *du = nullptr;
+  if (hipMalloc(&u, bytes) != hipSuccess) return;
+  if (hipMalloc(&du, bytes) != hipSuccess) {
+    (void)hipFree(u);
+    return;
+  }
+  (void)hipMemset(u, 0, bytes);
+  (void)hipMemset(du, 0, bytes);
+  hipStream_t st = current_stream();
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  const CoefConst<T> cf = coefs<T>(p);
+  const int n[3] = {p->nx, p->ny, p->nz};
+  for (int dir = 0; dir < 3; dir++) {
+    const int order = dir == 0 ? 0 : 1;
+    const int vec = dir == 0 && (p->nx * sizeof(T)) % 16 == 0;
+    std::vector<PartCfg> cands =
+        candidates(order, n[dir], vec, sizeof(T), true, p->desc[dir].nsys);
+    float best = 1e30f;
+    for (PartCfg c : cands) {
+      c.idx32 = p->cfg[dir].idx32;
+      if (dir == 0 && c.vec != 2) continue;  // x keeps the staged layout
+      auto run = [&]() {
+        return dir == 2 ? launch_part<T, 1>(c, cf, du, u, p->desc[dir], st)
+                        : launch_part<T, 0>(c, cf, du, du, p->desc[dir], st);
+      };
+      if (run() != hipSuccess) continue;
+      (void)hipEventRecord(e0, st);
+      for (int r = 0; r < 3; r++) (void)run();
+      (void)hipEventRecord(e1, st);
+      if (hipEventSynchronize(e1) != hipSuccess) break;
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) {
+        best = ms;
+        p->cfg[dir] = c;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(u);
+  (void)hipFree(du);
+  (void)hipGetLastError();
+}
+
+void tune_plan(tridAdiPlan_st *p) {
+  if (p->dbl)
+    tune_plan_t<double>(p);
+  else
+    tune_plan_t<float>(p);
 }
 
 tridStatus_t check_args(tridAdiPlan_t p, void *u, void *du) {
@@ -194,9 +406,20 @@ EXTERN_C tridStatus_t tridAdiPlanCreate(tridAdiPlan_t *plan, int nx, int ny,
     c.idx32 = idx32;
     p->cfg[dir] = c;
   }
-  p->text = "rhs<" + std::string(p->dbl ? "double" : "float") + "> x:" +
-            cfg_name(p->cfg[0], ts) + " y:" + cfg_name(p->cfg[1], ts) +
-            " z:" + cfg_name(p->cfg[2], ts);
+  if (autotune_mode() == 1) tune_plan(p);
+  // fused RHS + x sweep: whole-chunk lines in one wave, waves within a plane
+  {
+    const int R = p->cfg[0].R;
+    const int cps = 1 << p->cfg[0].cps_log2;
+    if (p->cfg[0].vec == 2 && cps <= kWave && cps * R == nx && ny % (kWave / cps) == 0 &&
+        (R == 4 || R == 8 || R == 16) && 256 % cps == 0) {
+      p->fused_R = R;
+      p->fused_cps_log2 = p->cfg[0].cps_log2;
+    }
+  }
+  p->text = std::string(p->fused_R ? "rhs+x(fused)<" : "rhs<") +
+            (p->dbl ? "double" : "float") + "> x:" + cfg_name(p->cfg[0], ts) +
+            " y:" + cfg_name(p->cfg[1], ts) + " z:" + cfg_name(p->cfg[2], ts);
   *plan = p;
   return TRID_STATUS_SUCCESS;
 }
@@ -227,7 +450,9 @@ EXTERN_C tridStatus_t tridAdiStage(tridAdiPlan_t p, void *u, void *du, int stage
   clear_error();
   tridStatus_t s = check_args(p, u, du);
   if (s) return s;
-  if (stage < 0 || stage > 3) return fail(TRID_STATUS_INVALID_VALUE, "stage must be 0..3");
+  if (stage < 0 || stage > 4) return fail(TRID_STATUS_INVALID_VALUE, "stage must be 0..4");
+  if (stage == 4 && !use_fused(p, u, du))
+    return fail(TRID_STATUS_INVALID_VALUE, "this plan/layout has no fused RHS + x stage");
   return stage_any(p, u, du, stage, current_stream());
 }
 
