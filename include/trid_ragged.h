@@ -1,0 +1,50 @@
+/*
+ * trid_ragged.h -- many independent tridiagonal systems of uneven length in
+ * one persistent kernel launch.
+ *
+ * No reference counterpart as an API: the reference solves one length per
+ * call (tridXmtsvStridedBatch) or one system per call (trid_scalarD,
+ * scalar/include/trid_cpu.h:84-98).  Here system k occupies elements
+ * [offsets[k], offsets[k+1]) of a, b, c, d (and u), contiguous, and is
+ * solved exactly as trid_scalarD would solve it: a of its first row and c
+ * of its last row are never read; non-Inc overwrites d, Inc adds into u.
+ *
+ * tridRaggedPlanCreate reads the offsets once (device array of nsys+1
+ * int64 values, non-decreasing), bins the systems by length into chunked
+ * schedules (one lane per short system; 2..64 lanes per system with an
+ * in-wave reduced solve; 64..256 lanes per system across waves through LDS)
+ * and builds a work list ordered from the most to the least expensive item.
+ * Each solve is one launch of a persistent kernel whose work-groups pull
+ * items from an atomic counter (reset by the kernel itself).  Systems longer
+ * than 4096 rows are solved by a second, one-thread-per-system launch.
+ * A plan may be used for any arrays with the same offsets; calls that
+ * overlap on one plan race on its counter and must be stream-ordered.
+ */
+#ifndef TRIDSOLVER_AMD_TRID_RAGGED_H
+#define TRIDSOLVER_AMD_TRID_RAGGED_H
+
+#include <stdint.h>
+
+#include "trid_common.h"
+
+typedef struct tridRaggedPlan_st *tridRaggedPlan_t;
+
+/* is_double: 1 = FP64, 0 = FP32 (the plan is bound to one precision). */
+EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
+                                           const int64_t *offsets,
+                                           long long nsys, int is_double);
+EXTERN_C tridStatus_t tridRaggedPlanDestroy(tridRaggedPlan_t plan);
+
+EXTERN_C tridStatus_t tridDgtsvRaggedBatch(tridRaggedPlan_t plan,
+                                           const double *a, const double *b,
+                                           const double *c, double *d,
+                                           double *u, int inc, int sync);
+EXTERN_C tridStatus_t tridSgtsvRaggedBatch(tridRaggedPlan_t plan,
+                                           const float *a, const float *b,
+                                           const float *c, float *d, float *u,
+                                           int inc, int sync);
+
+/* Summary of the plan's bins ("R8/cps4: 1234 systems, ..."). */
+EXTERN_C const char *tridRaggedPlanDescribe(tridRaggedPlan_t plan);
+
+#endif

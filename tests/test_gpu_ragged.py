@@ -1,0 +1,117 @@
+"""GPU parity of ragged batches (trid_ragged.h) against the oracle.
+
+Each system k of [offsets[k], offsets[k+1]) must equal the reference's 1-D
+solve of that system (oracle_ragged restates trid_scalarD per system),
+within 1e-12 (FP64) / 1e-5 (FP32) normwise, plus the reference tests'
+elementwise tolerance.  Covers every length bin (1..8, wave-level 2..64
+chunks, multi-wave up to 4096, the one-thread path beyond), empty systems,
+Inc, repeated solves with one plan (the persistent kernel's self-reset
+counter) and the benchmark configuration (200,000 systems, lengths
+log-uniform in [8, 4096], seed 0).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from tridsolver_amd.ragged import RaggedPlan
+from tests.helpers import check_close
+
+pytestmark = pytest.mark.gpu
+
+
+def ragged_inputs(lengths, dtype, seed):
+    rng = np.random.default_rng(seed)
+    off = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
+    n = int(off[-1])
+    a = rng.uniform(-1.0, -0.1, n)
+    c = rng.uniform(-1.0, -0.1, n)
+    b = np.abs(a) + np.abs(c) + rng.uniform(0.5, 1.5, n)
+    d = rng.uniform(-1.0, 1.0, n)
+    return off, [x.astype(dtype) for x in (a, b, c, d)]
+
+
+def torch_dt(dt):
+    import torch
+    return torch.float64 if dt == np.float64 else torch.float32
+
+
+def run(off, arrs, dtype, inc=False, u=None, reps=1):
+    import torch
+    t_off = torch.from_numpy(off).to("cuda")
+    plan = RaggedPlan(t_off, torch_dt(dtype))
+    ta, tb, tc, td = (torch.from_numpy(x).to("cuda") for x in arrs)
+    d0 = td.clone()
+    tu = torch.from_numpy(u).to("cuda") if u is not None else None
+    u0 = tu.clone() if tu is not None else None
+    outs = []
+    for _ in range(reps):
+        if inc:
+            tu.copy_(u0)
+        else:
+            td.copy_(d0)
+        plan.solve(ta, tb, tc, td, tu, inc)
+        outs.append((tu if inc else td).cpu().numpy())
+    return outs, plan
+
+
+EDGE = [1, 2, 3, 7, 8, 9, 15, 16, 17, 31, 33, 63, 64, 65, 100, 255, 256, 257, 511, 512, 513,
+        1000, 1023, 1024, 1025, 2047, 2048, 2049, 4095, 4096, 4097, 5000, 0, 1, 12]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("inc", [False, True])
+def test_ragged_every_bin(dtype, inc):
+    rng = np.random.default_rng(5)
+    lengths = np.array(EDGE * 3 + list(rng.integers(1, 700, 500)), dtype=np.int64)
+    rng.shuffle(lengths)
+    off, arrs = ragged_inputs(lengths, dtype, seed=1)
+    u = rng.uniform(-1, 1, int(off[-1])).astype(dtype) if inc else None
+    (got,), plan = run(off, arrs, dtype, inc, u)
+    exp_d = arrs[3].copy()
+    exp_u = u.copy() if inc else None
+    assert oracle.ragged(arrs[0], arrs[1], arrs[2], exp_d, exp_u, off, inc=inc) == 0
+    check_close(exp_u if inc else exp_d, got, dtype, f"ragged [{plan.describe()}]")
+
+
+def test_ragged_repeated_solves_one_plan():
+    lengths = np.array([5, 300, 1200, 17, 4000, 64, 9000] * 40, dtype=np.int64)
+    off, arrs = ragged_inputs(lengths, np.float64, seed=2)
+    outs, _ = run(off, arrs, np.float64, reps=4)
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])
+
+
+def test_ragged_empty_and_all_empty():
+    import torch
+    off = torch.zeros(5, dtype=torch.int64, device="cuda")
+    plan = RaggedPlan(off, torch.float64)
+    x = torch.zeros(1, dtype=torch.float64, device="cuda")
+    plan.solve(x, x, x, x)
+
+
+def test_ragged_rejects_bad_offsets():
+    import torch
+    from tridsolver_amd import TridError
+    off = torch.tensor([0, 5, 3], dtype=torch.int64, device="cuda")
+    with pytest.raises(TridError):
+        RaggedPlan(off, torch.float64)
+
+
+def benchmark_lengths(nsys=200_000, seed=0):
+    """lengths log-uniform in [8, 4096]: floor(exp(U(ln 8, ln 4097)))"""
+    rng = np.random.default_rng(seed)
+    return np.floor(np.exp(rng.uniform(np.log(8), np.log(4097), nsys))).astype(np.int64)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_ragged_benchmark_configuration(dtype):
+    lengths = benchmark_lengths()
+    off, arrs = ragged_inputs(lengths, dtype, seed=0)
+    (got,), plan = run(off, arrs, dtype)
+    exp = arrs[3].copy()
+    assert oracle.ragged(arrs[0], arrs[1], arrs[2], exp, None, off) == 0
+    r = check_close(exp, got, dtype, "ragged benchmark")
+    print(f"ragged 200k systems, {int(off[-1])} unknowns, {np.dtype(dtype).name}: rel err {r:.2e} "
+          f"[{plan.describe()}]")

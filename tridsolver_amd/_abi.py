@@ -46,6 +46,12 @@ SIGNATURES = {
     "tridDbtsvMixedBatch": (ST, [P, I, I]),
     "tridDbtsvAdjoint": (ST, [P, P, P, P, P, P, P, P, P, I, I, I, I]),
     "tridBlkForceLanes": (ST, [I]),
+    # trid_ragged.h
+    "tridRaggedPlanCreate": (ST, [C.POINTER(P), P, C.c_longlong, I]),
+    "tridRaggedPlanDestroy": (ST, [P]),
+    "tridDgtsvRaggedBatch": (ST, [P, P, P, P, P, P, I, I]),
+    "tridSgtsvRaggedBatch": (ST, [P, P, P, P, P, P, I, I]),
+    "tridRaggedPlanDescribe": (C.c_char_p, [P]),
 }
 
 

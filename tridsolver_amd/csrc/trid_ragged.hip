@@ -1,0 +1,401 @@
+// trid_ragged.hip -- ragged batches in one persistent launch
+// (include/trid_ragged.h).
+//
+// Every system is solved with the partitioned Thomas scheme of
+// trid_part.hpp (chunks of R rows per lane, reduced system by PCR), with the
+// number of chunks per system (cps) chosen per length bin.  A work item is
+// one work-group of 256 threads processing 256/cps systems of one bin.
+// Loads are staged per system group through LDS: the cps lanes of a group
+// read the system's contiguous span with consecutive lanes on consecutive
+// elements, then each lane takes its chunk from LDS (the same transpose as
+// the solvedim-0 path, for spans at arbitrary offsets).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/trid_hip_ext.h"
+#include "../../include/trid_ragged.h"
+#include "trid_part.hpp"
+#include "trid_runtime.hpp"
+
+namespace tridhip {
+
+constexpr int kRagThreads = 256;
+constexpr int kRagMaxLen = 4096;  // longest system of the persistent kernel
+
+struct RagItem {
+  int R;         // chunk length (8 or 16)
+  int cps_log2;  // chunks per system
+  int first;     // first entry of the item in the sorted system list
+  int count;     // systems in this item (<= 256 / cps)
+};
+
+template <typename T>
+struct RagArgs {
+  const T *a, *b, *c;
+  T *d, *u;
+  int inc;
+  const int64_t *offsets;
+  const int *sys;  // system ids, grouped by bin
+  const RagItem *items;
+  int nitems;
+  unsigned *counter;  // [0] next item, [1] finished work-groups
+};
+
+// Group-staged access: the cps lanes of a system group move the system's
+// span [off, off+L) between global memory and the group's LDS region with
+// consecutive lanes on consecutive elements.
+template <int R>
+__device__ __forceinline__ void rag_sync(int cps) {
+  if (cps > kWave)
+    __syncthreads();
+  else
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void rag_load(const T *__restrict__ X, long long off,
+                                         int L, int k, int cps, T *st,
+                                         T (&x)[R], T pad) {
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int e = r * cps + k;
+    if (e < L) st[e] = X[off + e];
+  }
+  rag_sync<R>(cps);
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int e = k * R + r;
+    x[r] = e < L ? st[e] : pad;
+  }
+  rag_sync<R>(cps);
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void rag_item(const RagArgs<T> &g, const RagItem &it,
+                                         T *smem) {
+  const int tid = threadIdx.x;
+  const int cps = 1 << it.cps_log2;
+  const int k = tid & (cps - 1);
+  const int slot = tid >> it.cps_log2;
+  const bool active = slot < it.count;
+  long long off = 0;
+  int L = 0;
+  if (active) {
+    const int id = g.sys[it.first + slot];
+    off = g.offsets[id];
+    L = (int)(g.offsets[id + 1] - off);
+  }
+  T *st = smem + slot * (cps * R);
+  T a[R], b[R], c[R], d[R];
+  rag_load<T, R>(g.a, off, L, k, cps, st, a, T(0));
+  rag_load<T, R>(g.b, off, L, k, cps, st, b, T(1));
+  rag_load<T, R>(g.c, off, L, k, cps, st, c, T(0));
+  rag_load<T, R>(g.d, off, L, k, cps, st, d, T(0));
+  const int row0 = k * R;
+  if (row0 == 0) a[0] = T(0);
+#pragma unroll
+  for (int r = 0; r < R; r++)
+    if (row0 + r == L - 1) c[r] = T(0);
+
+  Row<T> f, l;
+  chunk_reduce<T, R>(a, b, c, d, f, l);
+  T xf, xl;
+  if (cps <= kWave) {
+    reduced_wave<T>(f, l, k, cps, tid & (kWave - 1), xf, xl);
+  } else {
+    // the PCR exchange buffers reuse the staging region (6 values per
+    // thread <= R per thread); the loads above are complete (barrier)
+    reduced_lds<T>(f, l, k, cps, slot, kRagThreads >> it.cps_log2, smem, xf, xl);
+    __syncthreads();
+  }
+  T y[R];
+  chunk_expand<T, R>(a, c, d, xf, xl, y);
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int e = row0 + r;
+    if (e < L) st[e] = y[r];
+  }
+  rag_sync<R>(cps);
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int e = r * cps + k;
+    if (e < L) {
+      if (g.inc)
+        g.u[off + e] += st[e];
+      else
+        g.d[off + e] = st[e];
+    }
+  }
+  // the next item reuses the staging region
+  __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRagThreads) k_ragged(RagArgs<T> g) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *smem = reinterpret_cast<T *>(smem_raw);
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(g.counter, 1u);
+    __syncthreads();
+    const int i = s_item;
+    __syncthreads();
+    if (i >= g.nitems) break;
+    const RagItem it = g.items[i];
+    if (it.R == 8)
+      rag_item<T, 8>(g, it, smem);
+    else
+      rag_item<T, 16>(g, it, smem);
+  }
+  // the last work-group to finish resets the counters for the next launch
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned done = atomicAdd(g.counter + 1, 1u);
+    if (done == gridDim.x - 1) {
+      atomicExch(g.counter, 0u);
+      atomicExch(g.counter + 1, 0u);
+    }
+  }
+}
+
+// Systems longer than kRagMaxLen: one thread per system, forward sweep's
+// c'/d' in a workspace (ws_off[j] = start of system j's 2*L slots).
+template <typename T>
+__global__ void __launch_bounds__(64)
+    k_ragged_long(RagArgs<T> g, const int *ids, const long long *ws_off, int n,
+                  T *ws) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int id = ids[j];
+  const long long off = g.offsets[id];
+  const int L = (int)(g.offsets[id + 1] - off);
+  T *cs = ws + ws_off[j];
+  T *ds = cs + L;
+  T bb = rcp(g.b[off]);
+  T cc = L > 1 ? bb * g.c[off] : T(0);
+  T dd = bb * g.d[off];
+  cs[0] = cc;
+  ds[0] = dd;
+  for (int i = 1; i < L; i++) {
+    const T aa = g.a[off + i];
+    bb = rcp(g.b[off + i] - aa * cc);
+    dd = (g.d[off + i] - aa * dd) * bb;
+    cc = i < L - 1 ? bb * g.c[off + i] : T(0);
+    cs[i] = cc;
+    ds[i] = dd;
+  }
+  for (int i = L - 1; i >= 0; i--) {
+    if (i < L - 1) dd = ds[i] - cs[i] * dd;
+    if (g.inc)
+      g.u[off + i] += dd;
+    else
+      g.d[off + i] = dd;
+  }
+}
+
+}  // namespace tridhip
+
+using namespace tridhip;
+
+struct tridRaggedPlan_st {
+  int dbl = 1;
+  long long nsys = 0;
+  const int64_t *offsets = nullptr;  // caller's device array
+  int *sys = nullptr;
+  RagItem *items = nullptr;
+  int nitems = 0;
+  unsigned *counter = nullptr;
+  int grid = 0;
+  int *long_ids = nullptr;
+  long long *long_ws_off = nullptr;
+  int nlong = 0;
+  long long long_ws = 0;  // elements
+  size_t lds = 0;
+  std::string text;
+};
+
+namespace {
+
+void free_plan(tridRaggedPlan_st *p) {
+  if (!p) return;
+  (void)hipFree(p->sys);
+  (void)hipFree(p->items);
+  (void)hipFree(p->counter);
+  (void)hipFree(p->long_ids);
+  (void)hipFree(p->long_ws_off);
+  delete p;
+}
+
+template <typename T>
+tridStatus_t ragged_solve(tridRaggedPlan_st *p, const T *a, const T *b, const T *c,
+                          T *d, T *u, int inc, int sync) {
+  clear_error();
+  if (!p) return fail(TRID_STATUS_INVALID_VALUE, "ragged plan is NULL");
+  if ((sizeof(T) == 8) != (p->dbl == 1))
+    return fail(TRID_STATUS_INVALID_VALUE, "ragged plan precision differs from the call");
+  if (p->nsys == 0) return TRID_STATUS_SUCCESS;
+  if (!a || !b || !c || !d || (inc && !u))
+    return fail(TRID_STATUS_INVALID_VALUE, "NULL array");
+  hipStream_t st = current_stream();
+  RagArgs<T> g{a, b, c, d, u, inc ? 1 : 0, p->offsets, p->sys, p->items, p->nitems, p->counter};
+  if (p->nitems > 0) {
+    hipLaunchKernelGGL(k_ragged<T>, dim3(p->grid), dim3(kRagThreads), p->lds, st, g);
+    tridStatus_t s = check_hip(hipGetLastError(), "ragged kernel launch");
+    if (s) return s;
+  }
+  if (p->nlong > 0) {
+    T *ws = static_cast<T *>(workspace((size_t)p->long_ws * sizeof(T)));
+    if (!ws) return fail(TRID_STATUS_ALLOC_FAILED, "workspace allocation failed");
+    hipLaunchKernelGGL(k_ragged_long<T>, dim3((p->nlong + 63) / 64), dim3(64), 0, st, g,
+                       p->long_ids, p->long_ws_off, p->nlong, ws);
+    tridStatus_t s = check_hip(hipGetLastError(), "ragged long-system launch");
+    if (s) return s;
+  }
+  if (sync) return check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+  return TRID_STATUS_SUCCESS;
+}
+
+}  // namespace
+
+EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
+                                           const int64_t *offsets,
+                                           long long nsys, int is_double) {
+  clear_error();
+  if (!plan) return fail(TRID_STATUS_INVALID_VALUE, "plan pointer is NULL");
+  *plan = nullptr;
+  if (nsys < 0 || nsys > (1LL << 31) - 2)
+    return fail(TRID_STATUS_INVALID_VALUE, "nsys out of range");
+  if (nsys > 0 && !offsets) return fail(TRID_STATUS_INVALID_VALUE, "offsets is NULL");
+  auto *p = new (std::nothrow) tridRaggedPlan_st;
+  if (!p) return fail(TRID_STATUS_ALLOC_FAILED, "plan allocation failed");
+  p->dbl = is_double ? 1 : 0;
+  p->nsys = nsys;
+  p->offsets = offsets;
+  if (nsys == 0) {
+    p->text = "empty";
+    *plan = p;
+    return TRID_STATUS_SUCCESS;
+  }
+  std::vector<int64_t> off(nsys + 1);
+  tridStatus_t s = check_hip(hipMemcpy(off.data(), offsets, sizeof(int64_t) * (nsys + 1),
+                                       hipMemcpyDeviceToHost),
+                             "reading offsets");
+  if (s) {
+    free_plan(p);
+    return s;
+  }
+  for (long long k = 0; k < nsys; k++)
+    if (off[k + 1] < off[k]) {
+      free_plan(p);
+      return fail(TRID_STATUS_INVALID_VALUE, "offsets must be non-decreasing");
+    }
+  // bins: R = 8 for L <= 64*8 (cps 1..64), R = 16 for L <= 4096 (cps 64..256)
+  const int Rsmall = p->dbl ? 8 : 16;
+  auto bin_of = [&](long long L, int &R, int &cl) {
+    R = L <= 64LL * Rsmall ? Rsmall : 16;
+    long long ch = (L + R - 1) / R;
+    cl = 0;
+    while ((1LL << cl) < ch) cl++;
+  };
+  std::vector<std::vector<int>> bins(2 * 16);
+  std::vector<int> longs;
+  std::vector<long long> long_off;
+  long long lws = 0;
+  for (long long k = 0; k < nsys; k++) {
+    const long long L = off[k + 1] - off[k];
+    if (L == 0) continue;
+    if (L > kRagMaxLen) {
+      longs.push_back((int)k);
+      long_off.push_back(lws);
+      lws += 2 * L;
+      continue;
+    }
+    int R, cl;
+    bin_of(L, R, cl);
+    bins[(R == 16 ? 16 : 0) + cl].push_back((int)k);
+  }
+  // items, most expensive first: larger R*cps first
+  std::vector<int> sys;
+  std::vector<RagItem> items;
+  std::string desc;
+  for (int Rsel : {16, 8}) {
+    for (int cl = 15; cl >= 0; cl--) {
+      std::vector<int> &v = bins[(Rsel == 16 ? 16 : 0) + cl];
+      if (v.empty()) continue;
+      if (Rsel != Rsmall && Rsel != 16) continue;
+      const int per = std::max(1, kRagThreads >> cl);
+      for (size_t i = 0; i < v.size(); i += per) {
+        RagItem it;
+        it.R = Rsel;
+        it.cps_log2 = cl;
+        it.first = (int)sys.size() + (int)i;
+        it.count = (int)std::min<size_t>(per, v.size() - i);
+        items.push_back(it);
+      }
+      sys.insert(sys.end(), v.begin(), v.end());
+      char buf[96];
+      snprintf(buf, sizeof buf, "R%d/cps%d:%zu ", Rsel, 1 << cl, v.size());
+      desc += buf;
+    }
+  }
+  if (!longs.empty()) desc += "long(>4096):" + std::to_string(longs.size());
+  p->text = desc;
+  p->nitems = (int)items.size();
+  p->nlong = (int)longs.size();
+  p->long_ws = lws;
+  const size_t ts = p->dbl ? 8 : 4;
+  p->lds = (size_t)kRagThreads * 16 * ts;
+  auto up = [&](void **dst, const void *src, size_t bytes) -> tridStatus_t {
+    if (bytes == 0) return TRID_STATUS_SUCCESS;
+    tridStatus_t r = check_hip(hipMalloc(dst, bytes), "hipMalloc");
+    if (r) return r;
+    return check_hip(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+  };
+  s = up((void **)&p->sys, sys.data(), sys.size() * sizeof(int));
+  if (!s) s = up((void **)&p->items, items.data(), items.size() * sizeof(RagItem));
+  if (!s) s = up((void **)&p->long_ids, longs.data(), longs.size() * sizeof(int));
+  if (!s) s = up((void **)&p->long_ws_off, long_off.data(), long_off.size() * sizeof(long long));
+  if (!s) {
+    unsigned zero[2] = {0, 0};
+    s = up((void **)&p->counter, zero, sizeof zero);
+  }
+  if (s) {
+    free_plan(p);
+    return s;
+  }
+  // persistent grid: enough work-groups to fill every CU several times
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  p->grid = std::max(1, std::min(p->nitems, cus * 4));
+  *plan = p;
+  return TRID_STATUS_SUCCESS;
+}
+
+EXTERN_C tridStatus_t tridRaggedPlanDestroy(tridRaggedPlan_t plan) {
+  free_plan(plan);
+  return TRID_STATUS_SUCCESS;
+}
+
+EXTERN_C tridStatus_t tridDgtsvRaggedBatch(tridRaggedPlan_t plan,
+                                           const double *a, const double *b,
+                                           const double *c, double *d,
+                                           double *u, int inc, int sync) {
+  return ragged_solve<double>(plan, a, b, c, d, u, inc, sync);
+}
+
+EXTERN_C tridStatus_t tridSgtsvRaggedBatch(tridRaggedPlan_t plan,
+                                           const float *a, const float *b,
+                                           const float *c, float *d, float *u,
+                                           int inc, int sync) {
+  return ragged_solve<float>(plan, a, b, c, d, u, inc, sync);
+}
+
+EXTERN_C const char *tridRaggedPlanDescribe(tridRaggedPlan_t plan) {
+  return plan ? plan->text.c_str() : "";
+}
