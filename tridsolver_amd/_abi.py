@@ -52,6 +52,13 @@ SIGNATURES = {
     "tridDgtsvRaggedBatch": (ST, [P, P, P, P, P, P, I, I]),
     "tridSgtsvRaggedBatch": (ST, [P, P, P, P, P, P, I, I]),
     "tridRaggedPlanDescribe": (C.c_char_p, [P]),
+    # trid_adjoint.h
+    "tridDmtsvStridedBatchAdjoint": (ST, [P] * 9 + [I, I, IP, IP, I]),
+    "tridSmtsvStridedBatchAdjoint": (ST, [P] * 9 + [I, I, IP, IP, I]),
+    "tridVadiForward": (ST, [P, P, I, I, I, I, C.c_double, I]),
+    "tridVadiLossGrad": (ST, [P, P, P, I, I, I, I, C.c_double, I, C.POINTER(C.c_double), P, P]),
+    "tridVadiRhs": (ST, [P, P, P, I, I, I, C.c_double]),
+    "tridVadiSweep": (ST, [P, P, P, I, I, I, I, C.c_double, I, I]),
 }
 
 

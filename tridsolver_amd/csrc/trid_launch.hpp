@@ -18,9 +18,9 @@ template <typename T, int INC, int ORDER, int R, int VEC, typename I,
           typename Coef>
 __global__ void __launch_bounds__((R >= 16 && sizeof(T) == 8) ? 512 : 1024)
     k_part(Coef coef, T *__restrict__ Dd, T *__restrict__ U, SysDesc D,
-           int cps_log2, int spb_log2) {
+           int cps_log2, int spb_log2, T *__restrict__ Out) {
   part_block<T, INC, ORDER, R, VEC, I>(coef, Dd, U, D, cps_log2, spb_log2,
-                                       blockIdx.x);
+                                       blockIdx.x, Out);
 }
 
 struct PartCfg {
@@ -41,7 +41,7 @@ inline int ilog2(long long v) {
 
 template <typename T, int INC, int ORDER, int R, int VEC, typename Coef>
 hipError_t launch_part_t(const PartCfg &cfg, const Coef &coef, T *d, T *u,
-                         const SysDesc &D, hipStream_t st) {
+                         const SysDesc &D, hipStream_t st, T *out) {
   const int cps = 1 << cfg.cps_log2, spb = 1 << cfg.spb_log2;
   const int threads = cps * spb;
   const long long blocks = (D.nsys + spb - 1) / spb;
@@ -51,35 +51,50 @@ hipError_t launch_part_t(const PartCfg &cfg, const Coef &coef, T *d, T *u,
   if (cfg.idx32)
     hipLaunchKernelGGL((k_part<T, INC, ORDER, R, VEC, uint32_t, Coef>),
                        dim3((unsigned)blocks), dim3(threads), lds, st, coef, d,
-                       u, D, cfg.cps_log2, cfg.spb_log2);
+                       u, D, cfg.cps_log2, cfg.spb_log2, out);
   else
     hipLaunchKernelGGL((k_part<T, INC, ORDER, R, VEC, long long, Coef>),
                        dim3((unsigned)blocks), dim3(threads), lds, st, coef, d,
-                       u, D, cfg.cps_log2, cfg.spb_log2);
+                       u, D, cfg.cps_log2, cfg.spb_log2, out);
   return hipGetLastError();
 }
 
 // Launches the partitioned kernel for any configuration of the table.
 template <typename T, int INC, typename Coef>
 hipError_t launch_part(const PartCfg &cfg, const Coef &coef, T *d, T *u,
-                       const SysDesc &D, hipStream_t st) {
-#define TRID_CASE(ORD, RR, VV)                                                 \
-  if (cfg.order == ORD && cfg.R == RR && cfg.vec == VV)                        \
-    return launch_part_t<T, INC, ORD, RR, VV>(cfg, coef, d, u, D, st);
-  TRID_CASE(0, 4, 0)
-  TRID_CASE(0, 8, 0)
-  TRID_CASE(0, 16, 0)
-  TRID_CASE(0, 4, 1)
-  TRID_CASE(0, 8, 1)
-  TRID_CASE(0, 16, 1)
-  TRID_CASE(0, 4, 2)
-  TRID_CASE(0, 8, 2)
-  TRID_CASE(0, 16, 2)
-  TRID_CASE(1, 4, 0)
-  TRID_CASE(1, 8, 0)
-  TRID_CASE(1, 16, 0)
-#undef TRID_CASE
-  return hipErrorInvalidValue;
+                       const SysDesc &D, hipStream_t st, T *out = nullptr) {
+  if constexpr (Coef::kHook) {
+    // hooked coefficient loaders take the scalar (VEC 0) paths only
+#define TRID_CASE0(ORD, RR)                                                    \
+  if (cfg.order == ORD && cfg.R == RR)                                         \
+    return launch_part_t<T, INC, ORD, RR, 0>(cfg, coef, d, u, D, st, out);
+    TRID_CASE0(0, 4)
+    TRID_CASE0(0, 8)
+    TRID_CASE0(0, 16)
+    TRID_CASE0(1, 4)
+    TRID_CASE0(1, 8)
+    TRID_CASE0(1, 16)
+#undef TRID_CASE0
+    return hipErrorInvalidValue;
+  } else {
+  #define TRID_CASE(ORD, RR, VV)                                                 \
+    if (cfg.order == ORD && cfg.R == RR && cfg.vec == VV)                        \
+      return launch_part_t<T, INC, ORD, RR, VV>(cfg, coef, d, u, D, st, out);
+    TRID_CASE(0, 4, 0)
+    TRID_CASE(0, 8, 0)
+    TRID_CASE(0, 16, 0)
+    TRID_CASE(0, 4, 1)
+    TRID_CASE(0, 8, 1)
+    TRID_CASE(0, 16, 1)
+    TRID_CASE(0, 4, 2)
+    TRID_CASE(0, 8, 2)
+    TRID_CASE(0, 16, 2)
+    TRID_CASE(1, 4, 0)
+    TRID_CASE(1, 8, 0)
+    TRID_CASE(1, 16, 0)
+  #undef TRID_CASE
+    return hipErrorInvalidValue;
+  }
 }
 
 inline bool cfg_valid(const PartCfg &cfg, int N) {

@@ -341,6 +341,7 @@ __device__ __forceinline__ void store_chunk_staged(T *__restrict__ span,
 template <typename T>
 struct CoefArrays {
   static constexpr bool kArrays = true;
+  static constexpr bool kHook = false;
   const T *__restrict__ A;
   const T *__restrict__ B;
   const T *__restrict__ C;
@@ -348,7 +349,84 @@ struct CoefArrays {
 template <typename T>
 struct CoefConst {
   static constexpr bool kArrays = false;
+  static constexpr bool kHook = false;
   T a, b, c;
+};
+
+// Coefficient policies with their own chunk loader (scalar loads, any
+// ORDER, VEC 0 only).  load() fills a, b, c of rows row0..row0+R-1 of the
+// system at element offset off (stride s) in arrays with the layout of
+// SysDesc array 0; rows past N are identity rows.
+//   CoefTransposed: the transposed system A^T of tridiag(A, B, C): sub-
+//                   diagonal c_{i-1}, diagonal b_i, super-diagonal a_{i+1}.
+//   CoefKappa:      the variable-diffusivity ADI operator along a line:
+//                   face values kf_{i+1/2} = (K_i + K_{i+1})/2 and, at rows
+//                   interior to the grid, a_i = -lam kf_{i-1/2},
+//                   c_i = -lam kf_{i+1/2}, b_i = 1 - a_i - c_i (identity rows
+//                   elsewhere); with transpose, its transposed operator.
+template <typename T>
+struct CoefTransposed {
+  static constexpr bool kArrays = false;
+  static constexpr bool kHook = true;
+  const T *__restrict__ A;
+  const T *__restrict__ B;
+  const T *__restrict__ C;
+  template <int R, typename I>
+  __device__ __forceinline__ void load(int row0, int N, bool, I off, I s,
+                                       T (&a)[R], T (&b)[R], T (&c)[R]) const {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int row = row0 + r;
+      if (row < N) {
+        a[r] = row >= 1 ? ld_nt(at(C, off + (I)(row - 1) * s)) : T(0);
+        b[r] = ld_nt(at(B, off + (I)row * s));
+        c[r] = row + 1 < N ? ld_nt(at(A, off + (I)(row + 1) * s)) : T(0);
+      } else {
+        a[r] = T(0);
+        b[r] = T(1);
+        c[r] = T(0);
+      }
+    }
+  }
+};
+
+template <typename T>
+struct CoefKappa {
+  static constexpr bool kArrays = false;
+  static constexpr bool kHook = true;
+  const T *__restrict__ K;
+  T lam;
+  int transpose;
+  template <int R, typename I>
+  __device__ __forceinline__ void load(int row0, int N, bool line_in, I off, I s,
+                                       T (&a)[R], T (&b)[R], T (&c)[R]) const {
+    // kappa of rows row0-1 .. row0+R (cached loads: neighbours re-read them)
+    T k[R + 2];
+#pragma unroll
+    for (int r = 0; r < R + 2; r++) {
+      const int row = row0 - 1 + r;
+      k[r] = (row >= 0 && row < N) ? *at(K, off + (I)row * s) : T(0);
+    }
+    auto inner = [&](int row) { return line_in && row >= 1 && row + 2 <= N; };
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int row = row0 + r;
+      const T am = -lam * (T(0.5) * (k[r + 1] + k[r]));      // -lam kf_{row-1/2}
+      const T cp = -lam * (T(0.5) * (k[r + 1] + k[r + 2]));  // -lam kf_{row+1/2}
+      const bool in = inner(row);
+      b[r] = in ? T(1) - am - cp : T(1);
+      if (!transpose) {
+        a[r] = in ? am : T(0);
+        c[r] = in ? cp : T(0);
+      } else {
+        // (A^T)_{row,row-1} = c_{row-1} = -lam kf_{row-1/2} if row-1 is an
+        // interior row; (A^T)_{row,row+1} = a_{row+1} = -lam kf_{row+1/2}
+        a[r] = inner(row - 1) ? am : T(0);
+        c[r] = inner(row + 1) ? cp : T(0);
+        if (row >= N) b[r] = T(1);
+      }
+    }
+  }
 };
 
 // Block-wide partitioned solve of a strided batch.
@@ -369,7 +447,9 @@ template <typename T, int INC, int ORDER, int R, int VEC, typename I,
 __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
                                            T *__restrict__ U, const SysDesc &D,
                                            int cps_log2, int spb_log2,
-                                           long long block) {
+                                           long long block,
+                                           T *__restrict__ Out = nullptr) {
+  static_assert(!Coef::kHook || VEC == 0, "hooked coefficients load scalars");
   const int tid = threadIdx.x;
   const int cps = 1 << cps_log2;
   const int spb = 1 << spb_log2;
@@ -421,7 +501,18 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
   }
 
   // ---- load ------------------------------------------------------------
-  if constexpr (!Coef::kArrays) {
+  if constexpr (Coef::kHook) {
+    if (active) {
+      coef.template load<R, I>(row0, N, line_in, off[0], s0, a, b, c);
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        a[r] = T(0);
+        b[r] = T(1);
+        c[r] = T(0);
+      }
+    }
+  } else if constexpr (!Coef::kArrays) {
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int row = row0 + r;
@@ -519,13 +610,14 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
   chunk_expand<T, R>(a, c, d, xf, xl, y);
 
   // ---- store -----------------------------------------------------------
+  T *__restrict__ Dst = Out ? Out : Dd;
   if (VEC == 2) {
     const int lane = tid & (kWave - 1);
     if (wave_active) {
       if (INC)
         store_chunk_staged<T, R>(at(U, wave_off[4]), stage, lane, y, true);
       else
-        store_chunk_staged<T, R>(at(Dd, wave_off[3]), stage, lane, y, false);
+        store_chunk_staged<T, R>(at(Dst, wave_off[3]), stage, lane, y, false);
     }
   } else if (full) {
     if (INC) {
@@ -547,11 +639,11 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
     } else {
       I od = off[3] + (I)row0 * s3;
       if (VEC) {
-        store_chunk_vec<T, R>(at(Dd, od), y);
+        store_chunk_vec<T, R>(at(Dst, od), y);
       } else {
 #pragma unroll
         for (int r = 0; r < R; r++) {
-          st_nt(at(Dd, od), y[r]);
+          st_nt(at(Dst, od), y[r]);
           od += s3;
         }
       }
@@ -565,7 +657,7 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
           T *pu = at(U, off[4] + (I)row * s4);
           st_nt(pu, ld_nt(pu) + y[r]);
         } else {
-          st_nt(at(Dd, off[3] + (I)row * s3), y[r]);
+          st_nt(at(Dst, off[3] + (I)row * s3), y[r]);
         }
       }
     }
