@@ -2,8 +2,9 @@
 """Benchmark of the tridsolver_amd hot path on one MI355X.
 
     python bench.py [--steps K] [--warmup W] [--workload adi|plain] [--dtype f64|f32]
+                    [--no-extras] [--no-cpu]
 
-Prints ONE JSON line (rank 0).  Workloads (synthetic data, seeded):
+Prints ONE JSON line.  Headline workloads (synthetic data, seeded):
   adi    ADI heat diffusion (apps/adi), 256^3, lambda = 1, a = c = -lambda,
          b = 1 + 2 lambda, Dirichlet-zero boundary, interior u0 ~ U(0,1).
          One step = one ADI time step: 7-point right-hand side, then x, y and
@@ -16,9 +17,20 @@ bracketed by torch.cuda.synchronize(); per-kernel durations come from HIP
 events recorded on the library's stream around every launch.
 
 "roofline" reports the dominant kernel's achieved algorithmic bandwidth
-(bytes every launch must move at minimum, see ALGO_BYTES) against MI355X's
-8 TB/s HBM peak.  "cpu_baseline" times the CPU oracle (a C/OpenMP port of
-the reference CPU solver) on the host cores on a bounded sample.
+(bytes every launch must move at minimum) against MI355X's 8 TB/s HBM peak;
+"traffic" is the HBM bytes per launch that rocprofv3 counters measured for
+that kernel (profiles/traffic.json).  "cpu_baseline" times the CPU oracle (a
+C/OpenMP port of the reference CPU solver) on the host cores on a bounded
+sample.  "extras" (same line) holds the other measured workloads:
+  plain     256^3 solves per solvedim, FP64 and FP32;
+  ragged    200,000 systems, lengths floor(exp(U(ln 8, ln 4097))) (seed 0),
+            one persistent launch vs one strided-batch launch per distinct
+            length on length-grouped copies (gather outside the timing),
+            eager and captured in a HIP graph;
+  vadi_grad 128^3 FP64, 16 steps, checkpoint every 4: loss + gradient vs the
+            forward pass;
+  block     65,536 systems x 128 block rows, M in {2,3,4,5,8}, FP64, and all
+            five in one mixed-M launch.
 """
 from __future__ import annotations
 
@@ -32,6 +44,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "Gunknowns/s + %HBM peak at 256\u00b3 FP32/FP64; ragged & fwd+bwd vs plain"
 
 
 def parse():
@@ -45,13 +58,18 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--autotune", type=int, default=1)
+    ap.add_argument("--no-extras", action="store_true", help="headline workload only")
+    ap.add_argument("--extras", default="plain,ragged,vadi,block",
+                    help="comma-separated subset of plain,ragged,vadi,block")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (profiler calibration)")
     return ap.parse_args()
 
 
 def traffic_from_profiles(kernel_key: str):
-    """HBM bytes per launch measured by rocprofv3 --pmc (profiles/traffic.json)."""
+    """HBM bytes per launch of a kernel as measured by rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE (calibrated against a plain device copy), from profiles/traffic.json;
+    keys are "<workload>|<kernel>|<dtype>"."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p) as f:
@@ -189,6 +207,253 @@ def cpu_baseline_plain(args, np, host, dims, seconds):
                 sample=f"{reps} x (solvedim 0,1,2 on the full {n}^3 grid), oracle C/OpenMP port")
 
 
+# ---------------------------------------------------------------- workloads
+
+
+def ragged_lengths(nsys=200_000, seed=0):
+    """lengths log-uniform in [8, 4096]: floor(exp(U(ln 8, ln 4097)))"""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return np.floor(np.exp(rng.uniform(np.log(8), np.log(4097), nsys))).astype(np.int64)
+
+
+def ragged_arrays(lengths, seed=0):
+    """int64 offsets and FP64 a, b, c, d (a, c ~ U(-1,-0.1), b = |a|+|c|+U(0.5,1.5),
+    d ~ U(-1,1); drawn in the order a, c, b, d from numpy PCG64 `seed`)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    off = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
+    n = int(off[-1])
+    a = rng.uniform(-1.0, -0.1, n)
+    c = rng.uniform(-1.0, -0.1, n)
+    b = np.abs(a) + np.abs(c) + rng.uniform(0.5, 1.5, n)
+    d = rng.uniform(-1.0, 1.0, n)
+    return off, [a, b, c, d]
+
+
+def fourier_kappa(n, seed, modes=8, amp=0.3):
+    """kappa = exp(amp * g), g = sum of `modes` Fourier modes
+    A_m cos(2 pi k_m . x + phi_m), k_m integer in [-3, 3]^3, A_m ~ N(0, 1/modes),
+    phi_m ~ U(0, 2 pi), x = grid index / n; numpy PCG64 seed `seed`.
+    Returned with shape (nz, ny, nx), x fastest."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ks = rng.integers(-3, 4, (modes, 3))
+    A = rng.normal(0, 1 / np.sqrt(modes), modes)
+    ph = rng.uniform(0, 2 * np.pi, modes)
+    z, y, x = np.meshgrid(*(np.arange(n) / n,) * 3, indexing="ij")
+    g = np.zeros((n, n, n))
+    for m in range(modes):
+        g += A[m] * np.cos(2 * np.pi * (ks[m, 0] * x + ks[m, 1] * y + ks[m, 2] * z) + ph[m])
+    return np.exp(amp * g)
+
+
+def _timed(torch, fn, reps):
+    """mean GPU milliseconds of fn() over `reps` calls, each bracketed by
+    events on the null stream (the library's stream here)"""
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(reps)]
+    torch.cuda.synchronize()
+    for e0, e1 in ev:
+        e0.record()
+        fn()
+        e1.record()
+    torch.cuda.synchronize()
+    return sum(e0.elapsed_time(e1) for e0, e1 in ev) / reps
+
+
+def extra_plain(torch, np, ta, n=256, reps=10):
+    lib = ta.lib()
+    from tridsolver_amd.api import _iv
+    rng = np.random.default_rng(0)
+    shape = (n, n, n)
+    a = rng.uniform(-1.0, -0.1, shape)
+    c = rng.uniform(-1.0, -0.1, shape)
+    b = np.abs(a) + np.abs(c) + rng.uniform(0.5, 1.5, shape)
+    d = rng.uniform(-1.0, 1.0, shape)
+    dims = _iv([n, n, n])
+    out = {}
+    for name, dt, ws, fn in (("f64", np.float64, 8, lib.tridDmtsvStridedBatch),
+                             ("f32", np.float32, 4, lib.tridSmtsvStridedBatch)):
+        dev = [torch.from_numpy(x.astype(dt)).to("cuda") for x in (a, b, c, d)]
+        p = [t.data_ptr() for t in dev]
+        for sd in (0, 1, 2):
+            def solve():
+                st = fn(p[0], p[1], p[2], p[3], None, 3, sd, dims, dims, None, 0)
+                if st:
+                    raise RuntimeError(ta.last_error())
+            solve()
+            solve()
+            ms = _timed(torch, solve, reps)
+            gbs = 5 * ws * n ** 3 / (ms * 1e-3) / 1e9
+            out[f"{name}_solvedim{sd}"] = {
+                "ms": round(ms, 5), "Gunknowns_s": round(n ** 3 / (ms * 1e-3) / 1e9, 2),
+                "GB_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                "config": ta.last_config()}
+        del dev
+    return {"workload": f"{n}^3, one solve per solvedim, inputs as the plain workload",
+            "bytes_model": "5 words per unknown (read a, b, c, d; write d)", **out}
+
+
+def extra_ragged(torch, np, ta, reps=10):
+    import ctypes as C
+    from tridsolver_amd.ragged import RaggedPlan
+    lib = ta.lib()
+    lengths = ragged_lengths()
+    off, host = ragged_arrays(lengths)
+    n = int(off[-1])
+    nsys = len(lengths)
+    # length-grouped copy: systems sorted by length (stable), contiguous
+    order = np.argsort(lengths, kind="stable")
+    slen = lengths[order]
+    soff = np.concatenate([[0], np.cumsum(slen)])
+    idx = (np.arange(n, dtype=np.int64) - np.repeat(soff[:-1], slen)
+           + np.repeat(off[:-1][order], slen))
+    uniq, first = np.unique(slen, return_index=True)
+    counts = np.diff(np.append(first, nsys))
+    out = {"workload": f"{nsys} systems, lengths floor(exp(U(ln 8, ln 4097))) seed 0, "
+                       f"{n} unknowns, {len(uniq)} distinct lengths",
+           "bytes_model": "5 words per unknown + 8 B per system (offsets)"}
+    t_off = torch.from_numpy(off).to("cuda")
+    t_idx = torch.from_numpy(idx).to("cuda")
+    for name, tdt, ws, fn in (("f64", torch.float64, 8, lib.tridDmtsvStridedBatch),
+                              ("f32", torch.float32, 4, lib.tridSmtsvStridedBatch)):
+        dev = [torch.from_numpy(x).to("cuda").to(tdt) for x in host]
+        d0 = dev[3].clone()
+        plan = RaggedPlan(t_off, tdt)
+        srt = [x[t_idx] for x in dev]
+        sd0 = srt[3].clone()
+        # agreement of the two paths on the same systems
+        plan.solve(dev[0], dev[1], dev[2], dev[3])
+        p = [x.data_ptr() for x in srt]
+        dims = [(C.c_int * 2)(int(L), int(k)) for L, k in zip(uniq, counts)]
+        base = [int(s) * ws for s in soff[first]]
+
+        def per_length():
+            for j in range(len(uniq)):
+                o = base[j]
+                st = fn(p[0] + o, p[1] + o, p[2] + o, p[3] + o, None, 2, 0, dims[j], dims[j],
+                        None, 0)
+                if st:
+                    raise RuntimeError(ta.last_error())
+        per_length()
+        torch.cuda.synchronize()
+        diff = float((dev[3][t_idx] - srt[3]).abs().max() / srt[3].abs().max())
+        dev[3].copy_(d0)
+        srt[3].copy_(sd0)
+        t_rag = _timed(torch, lambda: plan.solve(dev[0], dev[1], dev[2], dev[3], sync=0), reps)
+        t_eager = _timed(torch, per_length, 3)
+        # the per-length launches captured once into a graph and replayed
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        t_graph = None
+        try:
+            with torch.cuda.graph(g, stream=s):
+                ta.set_stream(s)
+                per_length()
+            ta.set_stream(None)
+            g.replay()
+            torch.cuda.synchronize()
+            t_graph = _timed(torch, g.replay, reps)
+        finally:
+            ta.set_stream(None)
+        gbs = (5 * ws * n + 8 * (nsys + 1)) / (t_rag * 1e-3) / 1e9
+        out[name] = {
+            "ragged_ms": round(t_rag, 4), "Gunknowns_s": round(n / (t_rag * 1e-3) / 1e9, 2),
+            "GB_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "per_length_eager_ms": round(t_eager, 3),
+            "per_length_graph_ms": None if t_graph is None else round(t_graph, 3),
+            "speedup_vs_eager": round(t_eager / t_rag, 2),
+            "speedup_vs_graph": None if t_graph is None else round(t_graph / t_rag, 2),
+            "max_rel_diff_ragged_vs_per_length": diff,
+            "plan": plan.describe()}
+        plan.close()
+        del dev, srt, g
+        torch.cuda.empty_cache()
+    return out
+
+
+def extra_vadi(torch, np, ta, n=128, steps=16, ckpt=4, reps=5):
+    from tridsolver_amd.autograd import vadi_loss_grad
+    lib = ta.lib()
+    kap = torch.from_numpy(fourier_kappa(n, seed=0)).to("cuda")
+    rng = np.random.default_rng(0)
+    u0 = torch.from_numpy(rng.uniform(0, 1, (n, n, n))).to("cuda")
+    tgt = torch.from_numpy(rng.uniform(0, 1, (n, n, n))).to("cuda")
+    u = u0.clone()
+
+    def fwd():
+        u.copy_(u0)
+        st = lib.tridVadiForward(kap.data_ptr(), u.data_ptr(), n, n, n, steps, 1.0, 0)
+        if st:
+            raise RuntimeError(ta.last_error())
+    fwd()
+    t_copy = _timed(torch, lambda: u.copy_(u0), reps)
+    t_fwd = _timed(torch, fwd, reps) - t_copy
+    res = {}
+
+    def grad():
+        res["loss"] = vadi_loss_grad(kap, u0, tgt, steps, 1.0, ckpt)[0]
+    grad()
+    t_grad = _timed(torch, grad, reps)
+    return {"workload": f"variable-diffusivity ADI {n}^3 FP64, {steps} steps, lambda 1, kappa = "
+                        f"exp(0.3 g), g = 8 Fourier modes seed 0, checkpoint every {ckpt} steps",
+            "forward_ms": round(t_fwd, 4), "loss_and_grad_ms": round(t_grad, 4),
+            "grad_over_forward": round(t_grad / t_fwd, 3),
+            "forward_Gunknowns_s": round(3 * n ** 3 * steps / (t_fwd * 1e-3) / 1e9, 2),
+            "loss": res["loss"]}
+
+
+def block_inputs(torch, N, P, M, gen):
+    """Off-diagonal blocks U(-1,1)/M; B = U(-1,1)/M + (row-sum |A|+|C| + 1) I;
+    d ~ U(-1,1); A_0 and C_{N-1} zero; reference GPU layout (N, P, M, M)."""
+    def U(*shape):
+        return (torch.rand(*shape, generator=gen, device="cuda", dtype=torch.float64) * 2 - 1)
+    A = U(N, P, M, M) / M
+    Cm = U(N, P, M, M) / M
+    A[0].zero_()
+    Cm[N - 1].zero_()
+    B = U(N, P, M, M) / M
+    B += torch.diag_embed(A.abs().sum(-1) + Cm.abs().sum(-1) + 1.0)
+    d = U(N, P, M)
+    return A, B, Cm, d
+
+
+def extra_block(torch, np, ta, N=128, P=65536, reps=5):
+    from tridsolver_amd.block import btsv, btsv_mixed
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0)
+    out = {"workload": f"{P} systems x {N} block rows, FP64, blocks as the block tests",
+           "bytes_model": "(3M^2 + 2M) words per block row (read A, B, C, d; write u); "
+                          "ref_model adds the C'/d' scratch round trip "
+                          "((5N-3)M^2 + (5N-1)M words per system)"}
+    groups = []
+    t_sep = 0.0
+    for M in (2, 3, 4, 5, 8):
+        A, B, Cm, d = block_inputs(torch, N, P, M, gen)
+        u = torch.empty_like(d)
+        Cs = torch.empty_like(A)
+        ds = torch.empty_like(d)
+        btsv(A, B, Cm, d, u, Cs, ds)
+        ms = _timed(torch, lambda: btsv(A, B, Cm, d, u, Cs, ds, sync=0), reps)
+        t_sep += ms
+        algo = (3 * M * M + 2 * M) * N * P * 8
+        ref = ((5 * N - 3) * M * M + (5 * N - 1) * M) * P * 8
+        out[f"M{M}"] = {"ms": round(ms, 4),
+                        "Gunknowns_s": round(N * P * M / (ms * 1e-3) / 1e9, 2),
+                        "GB_s": round(algo / (ms * 1e-3) / 1e9, 1),
+                        "frac": round(algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "ref_model_GB_s": round(ref / (ms * 1e-3) / 1e9, 1),
+                        "ref_model_frac": round(ref / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        groups.append(dict(A=A, B=B, C=Cm, d=d, Cstar=Cs, dstar=ds, u=u))
+    ms = _timed(torch, lambda: btsv_mixed(groups, sync=0), reps)
+    out["mixed_all_M_one_launch_ms"] = round(ms, 4)
+    out["separate_launches_sum_ms"] = round(t_sep, 4)
+    del groups
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -248,7 +513,7 @@ def main():
             "config": W["configs"].get(k, ""),
         }
     out = {
-        "metric": "Gunknowns/s + %HBM peak at 256^3 FP32/FP64",
+        "metric": METRIC,
         "value": round(value, 3),
         "unit": "Gunknowns/s",
         "n_gpus": 1,
@@ -268,7 +533,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles(f"{W['workload']}|{W['kernels'][dom]}|{args.dtype}"),
+            "traffic": traffic_from_profiles(f"{args.workload}|{W['kernels'][dom]}|{args.dtype}"),
         },
         "kernels": kernels,
         "graph_ms_per_step": (round(W["graph_ms"](args.steps), 5) if "graph_ms" in W else None),
@@ -278,6 +543,17 @@ def main():
     if not args.no_cpu:
         fnb = cpu_baseline_adi if args.workload == "adi" else cpu_baseline_plain
         out["cpu_baseline"] = fnb(args, np, W["host"], W["dims"], args.cpu_seconds)
+    if not args.no_extras:
+        del W
+        torch.cuda.empty_cache()
+        fns = {"plain": extra_plain, "ragged": extra_ragged, "vadi": extra_vadi,
+               "block": extra_block}
+        out["extras"] = {}
+        for k in args.extras.split(","):
+            k = k.strip()
+            if k:
+                print(f"[bench] extra {k}", file=sys.stderr, flush=True)
+                out["extras"][k] = fns[k](torch, np, ta)
     print(json.dumps(out), flush=True)
 
 

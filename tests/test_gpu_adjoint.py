@@ -18,6 +18,7 @@ import numpy as np
 import pytest
 
 import oracle
+from bench import fourier_kappa
 from tests.helpers import dd_system, oracle_solve
 
 pytestmark = pytest.mark.gpu
@@ -60,21 +61,6 @@ def test_autograd_gradcheck():
         ts = [dev(v).reshape(shape).requires_grad_(True) for v in (a, b, c, d)]
         assert torch.autograd.gradcheck(lambda a_, b_, c_, d_: TridSolve.apply(a_, b_, c_, d_, sd),
                                         ts, eps=1e-6, atol=1e-8, rtol=1e-6)
-
-
-def fourier_kappa(n, seed, modes=8, amp=0.3):
-    """kappa = exp(amp * g), g = sum of `modes` Fourier modes
-    A_m cos(2 pi k_m . x + phi_m), k_m integer in [-3, 3]^3, A_m ~ N(0, 1/modes),
-    phi_m ~ U(0, 2 pi), x = grid index / n; numpy PCG64 seed `seed`."""
-    rng = np.random.default_rng(seed)
-    ks = rng.integers(-3, 4, (modes, 3))
-    A = rng.normal(0, 1 / np.sqrt(modes), modes)
-    ph = rng.uniform(0, 2 * np.pi, modes)
-    z, y, x = np.meshgrid(*(np.arange(n) / n,) * 3, indexing="ij")
-    g = np.zeros((n, n, n))
-    for m in range(modes):
-        g += A[m] * np.cos(2 * np.pi * (ks[m, 0] * x + ks[m, 1] * y + ks[m, 2] * z) + ph[m])
-    return np.exp(amp * g)
 
 
 @pytest.mark.parametrize("n,steps,ckpt", [(12, 5, 2), (16, 8, 4), (10, 3, 1), (9, 6, 6)])

@@ -16,6 +16,7 @@ import pytest
 
 import oracle
 from tridsolver_amd.ragged import RaggedPlan
+from bench import ragged_lengths
 from tests.helpers import check_close
 
 pytestmark = pytest.mark.gpu
@@ -99,15 +100,9 @@ def test_ragged_rejects_bad_offsets():
         RaggedPlan(off, torch.float64)
 
 
-def benchmark_lengths(nsys=200_000, seed=0):
-    """lengths log-uniform in [8, 4096]: floor(exp(U(ln 8, ln 4097)))"""
-    rng = np.random.default_rng(seed)
-    return np.floor(np.exp(rng.uniform(np.log(8), np.log(4097), nsys))).astype(np.int64)
-
-
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_ragged_benchmark_configuration(dtype):
-    lengths = benchmark_lengths()
+    lengths = ragged_lengths()
     off, arrs = ragged_inputs(lengths, dtype, seed=0)
     (got,), plan = run(off, arrs, dtype)
     exp = arrs[3].copy()
