@@ -37,10 +37,11 @@ EXTERN_C const char *tridGetLastConfig(void);
 /* Forces the next strided-batch calls onto one kernel configuration
  * (for tests and the autotuner's own sweeps).  kind: 0 = automatic,
  * 1 = partitioned Thomas with wave/LDS reduced-system PCR, 2 = one thread
- * per system with a global-memory forward sweep (the fallback).  rows is the
- * chunk length R of kind 1 (4, 8 or 16; 0 = automatic), cps the chunks per
- * system (power of two; 0 = automatic), spb the systems per work-group
- * (0 = automatic). */
+ * per system with a global-memory forward sweep (the fallback), 3 = kind 1
+ * with the XCD-contiguous work-group order (every XCD takes one contiguous
+ * range of systems).  rows is the chunk length R of kinds 1 and 3 (4, 8 or
+ * 16; 0 = automatic), cps the chunks per system (power of two;
+ * 0 = automatic), spb the systems per work-group (0 = automatic). */
 EXTERN_C tridStatus_t tridForceConfig(int kind, int rows, int cps, int spb);
 
 /* Diagnostic: y[i] = the solvers' reciprocal of x[i] for n device values

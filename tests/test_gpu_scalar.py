@@ -139,8 +139,10 @@ def test_a0_and_cN_are_never_read(sd):
     np.testing.assert_array_equal(r1, r2)
 
 
+# (kind, rows, cps, spb); kind 3 = kind 1 with the XCD-contiguous block order
+# (the [45, 38, 27] grid gives 65 blocks at spb 16: 64 remapped, 1 identity)
 CONFIGS = [(1, 4, 0, 0), (1, 8, 0, 0), (1, 16, 0, 0), (1, 8, 0, 16), (1, 4, 0, 64),
-           (1, 16, 0, 32), (2, 0, 0, 0)]
+           (1, 16, 0, 32), (2, 0, 0, 0), (3, 8, 0, 16), (3, 16, 0, 32), (3, 4, 0, 64)]
 
 
 @pytest.mark.parametrize("cfg", CONFIGS)

@@ -67,35 +67,6 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Wave-staged load of 64*R contiguous elements with ordinary (cached) loads:
-// u is read by five kernels' worth of neighbours, so it stays cacheable.
-template <typename T, int R>
-__device__ __forceinline__ void load_span_cached(const T *__restrict__ span,
-                                                 T *__restrict__ buf, int lane,
-                                                 T (&x)[R], T *left = nullptr,
-                                                 T *right = nullptr) {
-  using V = typename Vec16<T>::type;
-  constexpr int E = Vec16<T>::n;
-  const V *g = reinterpret_cast<const V *>(span);
-  V *s = reinterpret_cast<V *>(buf);
-  V t[R / E];
-#pragma unroll
-  for (int v = 0; v < R / E; v++) t[v] = g[v * kWave + lane];
-#pragma unroll
-  for (int v = 0; v < R / E; v++) s[v * kWave + lane] = t[v];
-  __builtin_amdgcn_wave_barrier();
-  const V *mine = reinterpret_cast<const V *>(buf + lane * R);
-#pragma unroll
-  for (int v = 0; v < R / E; v++) {
-    V w = mine[v];
-#pragma unroll
-    for (int e = 0; e < E; e++) x[E * v + e] = w[e];
-  }
-  if (left) *left = lane > 0 ? buf[lane * R - 1] : T(0);
-  if (right) *right = lane < kWave - 1 ? buf[lane * R + R] : T(0);
-  __builtin_amdgcn_wave_barrier();
-}
-
 // The 7-point sum in the reference's order (apps/adi/src/adi_orig.c:183-186),
 // without contraction.
 template <typename T>
@@ -153,13 +124,20 @@ __global__ void __launch_bounds__(256)
     d[r] = T(0);
   }
   if (wave_active && plane_in) {
+    // all five spans in flight before the first transpose
+    Span<T, R> sc, sym, syp, szm, szp;
+    span_issue<T, R>(u + wo, lane, sc);
+    span_issue<T, R>(u + wo - nx, lane, sym);
+    span_issue<T, R>(u + wo + nx, lane, syp);
+    span_issue<T, R>(u + wo - nxy, lane, szm);
+    span_issue<T, R>(u + wo + nxy, lane, szp);
     T uc[R], ul, ur;
-    load_span_cached<T, R>(u + wo, stage, lane, uc, &ul, &ur);
+    span_stage<T, R>(sc, stage, lane, uc, &ul, &ur);
     T ym[R], yp[R], zm[R], zp[R];
-    load_span_cached<T, R>(u + wo - nx, stage, lane, ym);
-    load_span_cached<T, R>(u + wo + nx, stage, lane, yp);
-    load_span_cached<T, R>(u + wo - nxy, stage, lane, zm);
-    load_span_cached<T, R>(u + wo + nxy, stage, lane, zp);
+    span_stage<T, R>(sym, stage, lane, ym);
+    span_stage<T, R>(syp, stage, lane, yp);
+    span_stage<T, R>(szm, stage, lane, zm);
+    span_stage<T, R>(szp, stage, lane, zp);
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int row = row0 + r;
@@ -222,8 +200,6 @@ struct tridAdiPlan_st {
 
 namespace {
 
-bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
-
 template <typename T>
 CoefConst<T> coefs(const tridAdiPlan_st *p) {
   const T lam = (T)p->lambda;
@@ -249,8 +225,6 @@ hipError_t launch_fused(tridAdiPlan_st *p, T *u, T *du, hipStream_t st) {
                        p->nx, p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2);
   return hipGetLastError();
 }
-
-bool aligned16(const void *p);
 
 // Stage 4 = fused RHS + x sweep (plans with fused_R > 0 only).
 template <typename T>
@@ -302,8 +276,24 @@ tridStatus_t steps_any(tridAdiPlan_st *p, void *u, void *du, int nsteps, hipStre
   return TRID_STATUS_SUCCESS;
 }
 
+// The fused kernel needs nx == cps * R with cps <= 64 a power of two and ny
+// a multiple of the lines per wave (64 / cps).
+bool fused_rows_ok(const tridAdiPlan_st *p, int R) {
+  if (R != 4 && R != 8 && R != 16) return false;
+  if (p->nx % R) return false;
+  const int cps = p->nx / R;
+  if (cps > kWave || (cps & (cps - 1))) return false;
+  return p->ny % (kWave / cps) == 0;
+}
+
+void set_fused(tridAdiPlan_st *p, int R) {
+  p->fused_R = R;
+  p->fused_cps_log2 = ilog2(p->nx / R);
+}
+
 // Times every candidate configuration of the y and z sweeps (and of the x
-// sweep) on scratch grids and keeps the fastest.  Data values do not change
+// sweep, and the chunk length of the fused RHS + x kernel) on scratch grids
+// and keeps the fastest.  Data values do not change
 // the kernels' work, so zero-filled scratch is enough.
 template <typename T>
 void tune_plan_t(tridAdiPlan_st *p) {
@@ -317,39 +307,35 @@ void tune_plan_t(tridAdiPlan_st *p) {
   (void)hipMemset(u, 0, bytes);
   (void)hipMemset(du, 0, bytes);
   hipStream_t st = current_stream();
-  hipEvent_t e0, e1;
-  (void)hipEventCreate(&e0);
-  (void)hipEventCreate(&e1);
   const CoefConst<T> cf = coefs<T>(p);
   const int n[3] = {p->nx, p->ny, p->nz};
   for (int dir = 0; dir < 3; dir++) {
     const int order = dir == 0 ? 0 : 1;
     const int vec = dir == 0 && (p->nx * sizeof(T)) % 16 == 0;
-    std::vector<PartCfg> cands =
-        candidates(order, n[dir], vec, sizeof(T), true, p->desc[dir].nsys);
-    float best = 1e30f;
-    for (PartCfg c : cands) {
-      c.idx32 = p->cfg[dir].idx32;
+    std::vector<PartCfg> cands;
+    for (PartCfg c : candidates(order, n[dir], vec, sizeof(T), true, p->desc[dir].nsys)) {
       if (dir == 0 && c.vec != 2) continue;  // x keeps the staged layout
-      auto run = [&]() {
-        return dir == 2 ? launch_part<T, 1>(c, cf, du, u, p->desc[dir], st)
-                        : launch_part<T, 0>(c, cf, du, du, p->desc[dir], st);
-      };
-      if (run() != hipSuccess) continue;
-      (void)hipEventRecord(e0, st);
-      for (int r = 0; r < 3; r++) (void)run();
-      (void)hipEventRecord(e1, st);
-      if (hipEventSynchronize(e1) != hipSuccess) break;
-      float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      if (ms < best) {
-        best = ms;
-        p->cfg[dir] = c;
-      }
+      c.idx32 = p->cfg[dir].idx32;
+      cands.push_back(c);
     }
+    auto run = [&](int i) {
+      return dir == 2 ? launch_part<T, 1>(cands[i], cf, du, u, p->desc[dir], st)
+                      : launch_part<T, 0>(cands[i], cf, du, du, p->desc[dir], st);
+    };
+    const int i = pick_fastest((int)cands.size(), [](int) { return true; }, run, st);
+    if (i >= 0) p->cfg[dir] = cands[i];
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (p->fused_R) {
+    std::vector<int> rows;
+    for (int R : {4, 8, 16})
+      if (fused_rows_ok(p, R)) rows.push_back(R);
+    auto run = [&](int i) {
+      set_fused(p, rows[i]);
+      return launch_fused<T>(p, u, du, st);
+    };
+    const int i = pick_fastest((int)rows.size(), [](int) { return true; }, run, st);
+    set_fused(p, i >= 0 ? rows[i] : p->fused_R);
+  }
   (void)hipStreamSynchronize(st);
   (void)hipFree(u);
   (void)hipFree(du);
@@ -406,18 +392,17 @@ EXTERN_C tridStatus_t tridAdiPlanCreate(tridAdiPlan_t *plan, int nx, int ny,
     c.idx32 = idx32;
     p->cfg[dir] = c;
   }
-  if (autotune_mode() == 1) tune_plan(p);
-  // fused RHS + x sweep: whole-chunk lines in one wave, waves within a plane
-  {
-    const int R = p->cfg[0].R;
-    const int cps = 1 << p->cfg[0].cps_log2;
-    if (p->cfg[0].vec == 2 && cps <= kWave && cps * R == nx && ny % (kWave / cps) == 0 &&
-        (R == 4 || R == 8 || R == 16) && 256 % cps == 0) {
-      p->fused_R = R;
-      p->fused_cps_log2 = p->cfg[0].cps_log2;
-    }
+  // fused RHS + x sweep: whole-chunk lines in one wave, waves within a plane;
+  // by default with the x sweep's chunk length, else the first that fits
+  if ((nx * ts) % 16 == 0) {
+    for (int R : {p->cfg[0].R, 8, 4, 16})
+      if (fused_rows_ok(p, R)) {
+        set_fused(p, R);
+        break;
+      }
   }
-  p->text = std::string(p->fused_R ? "rhs+x(fused)<" : "rhs<") +
+  if (autotune_mode() == 1) tune_plan(p);
+  p->text = std::string(p->fused_R ? "rhs+x(fused,R" + std::to_string(p->fused_R) + ")<" : "rhs<") +
             (p->dbl ? "double" : "float") + "> x:" + cfg_name(p->cfg[0], ts) +
             " y:" + cfg_name(p->cfg[1], ts) + " z:" + cfg_name(p->cfg[2], ts);
   *plan = p;

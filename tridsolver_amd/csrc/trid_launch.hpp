@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -18,9 +19,18 @@ template <typename T, int INC, int ORDER, int R, int VEC, typename I,
           typename Coef>
 __global__ void __launch_bounds__((R >= 16 && sizeof(T) == 8) ? 512 : 1024)
     k_part(Coef coef, T *__restrict__ Dd, T *__restrict__ U, SysDesc D,
-           int cps_log2, int spb_log2, T *__restrict__ Out) {
+           int cps_log2, int spb_log2, int xcd, T *__restrict__ Out) {
+  // xcd: consecutive hardware blocks go round-robin to the 8 XCDs (each with
+  // its own L2); this remap hands every XCD one contiguous range of systems
+  // instead (a bijection on the first 8*floor(nblocks/8) blocks, identity on
+  // the rest).
+  long long block = blockIdx.x;
+  if (xcd) {
+    const unsigned nb8 = gridDim.x & ~7u;
+    if (blockIdx.x < nb8) block = (long long)(blockIdx.x & 7u) * (nb8 >> 3) + (blockIdx.x >> 3);
+  }
   part_block<T, INC, ORDER, R, VEC, I>(coef, Dd, U, D, cps_log2, spb_log2,
-                                       blockIdx.x, Out);
+                                       block, Out);
 }
 
 struct PartCfg {
@@ -31,7 +41,12 @@ struct PartCfg {
   int order = 0;
   int vec = 0;
   int idx32 = 1;  // 32-bit element offsets (every array < 4 GiB)
+  int xcd = 0;    // XCD-contiguous block order (see k_part)
 };
+
+inline bool aligned16(const void *p) {
+  return (reinterpret_cast<uintptr_t>(p) & 15) == 0;
+}
 
 inline int ilog2(long long v) {
   int l = 0;
@@ -51,11 +66,11 @@ hipError_t launch_part_t(const PartCfg &cfg, const Coef &coef, T *d, T *u,
   if (cfg.idx32)
     hipLaunchKernelGGL((k_part<T, INC, ORDER, R, VEC, uint32_t, Coef>),
                        dim3((unsigned)blocks), dim3(threads), lds, st, coef, d,
-                       u, D, cfg.cps_log2, cfg.spb_log2, out);
+                       u, D, cfg.cps_log2, cfg.spb_log2, cfg.xcd, out);
   else
     hipLaunchKernelGGL((k_part<T, INC, ORDER, R, VEC, long long, Coef>),
                        dim3((unsigned)blocks), dim3(threads), lds, st, coef, d,
-                       u, D, cfg.cps_log2, cfg.spb_log2, out);
+                       u, D, cfg.cps_log2, cfg.spb_log2, cfg.xcd, out);
   return hipGetLastError();
 }
 
@@ -138,9 +153,10 @@ inline std::string cfg_name(const PartCfg &c, size_t tsize) {
   if (c.kind == 2) {
     snprintf(buf, sizeof buf, "fallback<%s>", tsize == 8 ? "double" : "float");
   } else {
-    snprintf(buf, sizeof buf, "part<%s,o%d,R%d,cps%d,spb%d%s>",
+    snprintf(buf, sizeof buf, "part<%s,o%d,R%d,cps%d,spb%d%s%s>",
              tsize == 8 ? "double" : "float", c.order, c.R, 1 << c.cps_log2,
-             1 << c.spb_log2, c.vec == 2 ? ",staged" : (c.vec ? ",vec" : ""));
+             1 << c.spb_log2, c.vec == 2 ? ",staged" : (c.vec ? ",vec" : ""),
+             c.xcd ? ",xcd" : "");
   }
   return buf;
 }
@@ -153,21 +169,68 @@ inline std::vector<PartCfg> candidates(int order, int N, int vec, size_t tsize,
   auto add = [&](const PartCfg &c) {
     for (auto &o : out)
       if (o.R == c.R && o.cps_log2 == c.cps_log2 && o.spb_log2 == c.spb_log2 &&
-          o.vec == c.vec)
+          o.vec == c.vec && o.xcd == c.xcd)
         return;
     out.push_back(c);
+  };
+  auto add_both = [&](PartCfg c) {
+    add(c);
+    // the XCD-contiguous order only matters with several blocks per XCD
+    if ((nsys >> c.spb_log2) >= 64) {
+      c.xcd = 1;
+      add(c);
+    }
   };
   for (int R : Rs)
     for (int t : thr) {
       PartCfg c = make_cfg(order, R, N, vec && (R * (int)tsize) % 16 == 0, t);
       if (!cfg_valid(c, N)) continue;
-      add(c);
+      add_both(c);
       if (stage_ok(c, dense, nsys, N)) {
         c.vec = 2;
-        add(c);
+        add_both(c);
       }
     }
   return out;
+}
+
+// Autotuner timing: `passes` interleaved rounds over the candidates; in each
+// round candidate i gets prep(i) (untimed), one warm-up run(i) and `reps`
+// timed run(i) launches on st.  Returns the index with the lowest time of
+// its best round (-1 if no candidate could be launched).
+template <typename Prep, typename Run>
+int pick_fastest(int ncand, Prep prep, Run run, hipStream_t st, int reps = 5,
+                 int passes = 2) {
+  std::vector<float> best(ncand, 1e30f);
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return -1;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return -1;
+  }
+  for (int pass = 0; pass < passes; pass++) {
+    for (int i = 0; i < ncand; i++) {
+      if (!prep(i)) continue;
+      if (run(i) != hipSuccess) {
+        (void)hipGetLastError();
+        continue;
+      }
+      (void)hipEventRecord(e0, st);
+      for (int r = 0; r < reps; r++) (void)run(i);
+      (void)hipEventRecord(e1, st);
+      if (hipEventSynchronize(e1) != hipSuccess) break;
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      best[i] = std::min(best[i], ms);
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipGetLastError();
+  int arg = -1;
+  for (int i = 0; i < ncand; i++)
+    if (best[i] < 1e29f && (arg < 0 || best[i] < best[arg])) arg = i;
+  return arg;
 }
 
 inline PartCfg heuristic(int order, int N, int vec, size_t tsize) {

@@ -304,11 +304,14 @@ __device__ __forceinline__ void load_chunk_staged(const T *__restrict__ span,
   __builtin_amdgcn_wave_barrier();
 }
 
+// add: the span receives its own previous contents (or, with src, those of
+// src) plus x.
 template <typename T, int R>
 __device__ __forceinline__ void store_chunk_staged(T *__restrict__ span,
                                                    T *__restrict__ buf,
                                                    int lane, const T (&x)[R],
-                                                   bool add) {
+                                                   bool add,
+                                                   const T *src = nullptr) {
   using V = typename Vec16<T>::type;
   constexpr int E = Vec16<T>::n;
   V *mine = reinterpret_cast<V *>(buf + lane * R);
@@ -321,14 +324,70 @@ __device__ __forceinline__ void store_chunk_staged(T *__restrict__ span,
   }
   __builtin_amdgcn_wave_barrier();
   V *g = reinterpret_cast<V *>(span);
+  const V *gs = src ? reinterpret_cast<const V *>(src) : g;
   const V *s = reinterpret_cast<const V *>(buf);
 #pragma unroll
   for (int v = 0; v < R / E; v++) {
     V w = s[v * kWave + lane];
-    if (add) w += ld_nt(g + v * kWave + lane);
+    if (add) w += ld_nt(gs + v * kWave + lane);
     st_nt(g + v * kWave + lane, w);
   }
   __builtin_amdgcn_wave_barrier();
+}
+
+// Wave-staged load of 64*R contiguous elements with ordinary (cached) loads:
+// a stencil re-reads the span as a neighbour of other lines, so it stays
+// cacheable.  Split in two so that a kernel can have the global loads of
+// several spans in flight before it transposes them: span_issue moves the
+// span into registers (16 bytes per lane per instruction, consecutive lanes
+// on consecutive pieces), span_stage hands every lane its own R elements
+// through the wave's LDS buffer.  left/right: the elements just before and
+// after the lane's chunk within the span (0 outside it).
+template <typename T, int R>
+struct Span {
+  typename Vec16<T>::type v[R / Vec16<T>::n];
+};
+
+template <typename T, int R>
+__device__ __forceinline__ void span_issue(const T *__restrict__ span, int lane,
+                                           Span<T, R> &s) {
+  using V = typename Vec16<T>::type;
+  constexpr int E = Vec16<T>::n;
+  const V *g = reinterpret_cast<const V *>(span);
+#pragma unroll
+  for (int v = 0; v < R / E; v++) s.v[v] = g[v * kWave + lane];
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void span_stage(const Span<T, R> &t, T *__restrict__ buf,
+                                           int lane, T (&x)[R], T *left = nullptr,
+                                           T *right = nullptr) {
+  using V = typename Vec16<T>::type;
+  constexpr int E = Vec16<T>::n;
+  V *s = reinterpret_cast<V *>(buf);
+#pragma unroll
+  for (int v = 0; v < R / E; v++) s[v * kWave + lane] = t.v[v];
+  __builtin_amdgcn_wave_barrier();
+  const V *mine = reinterpret_cast<const V *>(buf + lane * R);
+#pragma unroll
+  for (int v = 0; v < R / E; v++) {
+    V w = mine[v];
+#pragma unroll
+    for (int e = 0; e < E; e++) x[E * v + e] = w[e];
+  }
+  if (left) *left = lane > 0 ? buf[lane * R - 1] : T(0);
+  if (right) *right = lane < kWave - 1 ? buf[lane * R + R] : T(0);
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void load_span_cached(const T *__restrict__ span,
+                                                 T *__restrict__ buf, int lane,
+                                                 T (&x)[R], T *left = nullptr,
+                                                 T *right = nullptr) {
+  Span<T, R> t;
+  span_issue<T, R>(span, lane, t);
+  span_stage<T, R>(t, buf, lane, x, left, right);
 }
 
 // Coefficient sources of part_block.
@@ -610,12 +669,16 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
   chunk_expand<T, R>(a, c, d, xf, xl, y);
 
   // ---- store -----------------------------------------------------------
+  // non-Inc: the solution goes to Out (default d); Inc: Out (default u)
+  // receives u + solution
   T *__restrict__ Dst = Out ? Out : Dd;
+  T *__restrict__ Udst = Out ? Out : U;
   if (VEC == 2) {
     const int lane = tid & (kWave - 1);
     if (wave_active) {
       if (INC)
-        store_chunk_staged<T, R>(at(U, wave_off[4]), stage, lane, y, true);
+        store_chunk_staged<T, R>(at(Udst, wave_off[4]), stage, lane, y, true,
+                                 at(static_cast<const T *>(U), wave_off[4]));
       else
         store_chunk_staged<T, R>(at(Dst, wave_off[3]), stage, lane, y, false);
     }
@@ -627,12 +690,11 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
         load_chunk_vec<T, R>(at(U, ou), u);
 #pragma unroll
         for (int r = 0; r < R; r++) u[r] += y[r];
-        store_chunk_vec<T, R>(at(U, ou), u);
+        store_chunk_vec<T, R>(at(Udst, ou), u);
       } else {
 #pragma unroll
         for (int r = 0; r < R; r++) {
-          T *pu = at(U, ou);
-          st_nt(pu, ld_nt(pu) + y[r]);
+          st_nt(at(Udst, ou), ld_nt(at(U, ou)) + y[r]);
           ou += s4;
         }
       }
@@ -654,8 +716,8 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
       const int row = row0 + r;
       if (row < N) {
         if (INC) {
-          T *pu = at(U, off[4] + (I)row * s4);
-          st_nt(pu, ld_nt(pu) + y[r]);
+          const I ou = off[4] + (I)row * s4;
+          st_nt(at(Udst, ou), ld_nt(at(U, ou)) + y[r]);
         } else {
           st_nt(at(Dst, off[3] + (I)row * s3), y[r]);
         }

@@ -147,41 +147,20 @@ PartCfg autotune(int order, int vec, int idx32, bool dense, const T *a,
   T *scratch = static_cast<T *>(workspace((size_t)d_extent * sizeof(T)));
   if (!scratch) return best;
   T *src = INC ? u : d;
-  hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess) return best;
-  if (hipEventCreate(&e1) != hipSuccess) {
-    (void)hipEventDestroy(e0);
-    return best;
-  }
-  float best_ms = 1e30f;
-  for (const PartCfg &cf : cands) {
-    if (hipMemcpyAsync(scratch, src, (size_t)d_extent * sizeof(T),
-                       hipMemcpyDeviceToDevice, st) != hipSuccess)
-      break;
-    T *dd = INC ? d : scratch;
-    T *uu = INC ? scratch : u;
-    if (launch_part<T, INC>(cf, CoefArrays<T>{a, b, c}, dd, uu, D, st) != hipSuccess)
-      continue;
-    (void)hipEventRecord(e0, st);
-    const int reps = 3;
-    for (int r = 0; r < reps; r++)
-      (void)launch_part<T, INC>(cf, CoefArrays<T>{a, b, c}, dd, uu, D, st);
-    (void)hipEventRecord(e1, st);
-    if (hipEventSynchronize(e1) != hipSuccess) break;
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = cf;
-    }
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipGetLastError();
+  T *dd = INC ? d : scratch;
+  T *uu = INC ? scratch : u;
+  // every candidate starts from a fresh copy of the written array
+  auto prep = [&](int) {
+    return hipMemcpyAsync(scratch, src, (size_t)d_extent * sizeof(T),
+                          hipMemcpyDeviceToDevice, st) == hipSuccess;
+  };
+  auto run = [&](int i) {
+    return launch_part<T, INC>(cands[i], CoefArrays<T>{a, b, c}, dd, uu, D, st);
+  };
+  const int i = pick_fastest((int)cands.size(), prep, run, st);
+  if (i >= 0) best = cands[i];
   return best;
 }
-
-bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
 template <typename T, int INC>
 tridStatus_t solve(const T *a, const int *a_pads, const T *b,
@@ -260,7 +239,7 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
   ForcedConfig fc = forced_config();
   if (fc.kind == 2) {
     cfg.kind = 2;
-  } else if (fc.kind == 1) {
+  } else if (fc.kind == 1 || fc.kind == 3) {
     int R = fc.rows ? fc.rows : (sizeof(T) == 8 ? 8 : 16);
     cfg = make_cfg(order, R, D.N, vec && (R * (int)sizeof(T)) % 16 == 0,
                    order == 0 ? 256 : 512);
@@ -269,6 +248,7 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
     if (!cfg_valid(cfg, D.N))
       return fail(TRID_STATUS_INVALID_VALUE, "forced configuration does not fit this shape");
     if (fc.stage && stage_ok(cfg, dense, D.nsys, D.N)) cfg.vec = 2;
+    cfg.xcd = fc.kind == 3;
   } else if (autotune_mode() == 1) {
     auto key = std::make_tuple((int)sizeof(T), INC, order, D.N, D.nsys,
                                vec * 2 + (int)dense);
