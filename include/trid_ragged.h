@@ -11,12 +11,13 @@
  *
  * tridRaggedPlanCreate reads the offsets once (device array of nsys+1
  * int64 values, non-decreasing), bins the systems by length into chunked
- * schedules (one lane per short system; 2..64 lanes per system with an
- * in-wave reduced solve; 64..256 lanes per system across waves through LDS)
- * and builds a work list ordered from the most to the least expensive item.
- * Each solve is one launch of a persistent kernel whose work-groups pull
- * items from an atomic counter (reset by the kernel itself).  Systems longer
- * than 4096 rows are solved by a second, one-thread-per-system launch.
+ * schedules (chunks of 8 rows; one lane per system up to 8 rows, 2..64
+ * lanes per system with an in-wave reduced solve, 128..512 lanes per system
+ * across the waves of a work-group through LDS) and builds a work list
+ * ordered from the most to the least expensive item.  Each solve is one
+ * launch of a persistent kernel whose work-groups pull items from an atomic
+ * counter (reset by the kernel itself).  Systems longer than 4096 rows are
+ * solved by a second, one-thread-per-system launch.
  * A plan may be used for any arrays with the same offsets; calls that
  * overlap on one plan race on its counter and must be stream-ordered.
  */

@@ -78,7 +78,15 @@ def test_adi_reference_mode_vs_reference_gold(name):
     assert err <= tol(dt)
 
 
-@pytest.mark.parametrize("shape", [(32, 32, 32), (33, 17, 9), (64, 40, 24), (8, 200, 6), (128, 64, 48)])
+# y and z lines of at most 512 rows take the factor-reuse sweeps (precomputed
+# constant-coefficient factors); (6, 600, 5) and (5, 6, 700) exercise the
+# generic sweep kernels along y and z; short lines (3, 5, 9 rows) the
+# single-chunk and identity-padded factor layouts
+ADI_SHAPES = [(32, 32, 32), (33, 17, 9), (64, 40, 24), (8, 200, 6), (128, 64, 48),
+              (6, 600, 5), (5, 6, 700), (16, 3, 5), (20, 9, 3)]
+
+
+@pytest.mark.parametrize("shape", ADI_SHAPES)
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_adi_vs_oracle(shape, dt, mode):

@@ -20,6 +20,7 @@
 
 #include "../../include/trid_adi.h"
 #include "../../include/trid_hip_ext.h"
+#include "trid_cc.hpp"
 #include "trid_launch.hpp"
 #include "trid_runtime.hpp"
 
@@ -192,6 +193,13 @@ struct tridAdiPlan_st {
   PartCfg cfg_x_unaligned;
   int fused_R = 0;          // >0: fused RHS + x sweep with this chunk length
   int fused_cps_log2 = 0;
+  // y and z sweeps with precomputed factors (trid_cc.hpp): chunk length (0 =
+  // the generic kernel), chunks per system and the factor table
+  int cc_R[3] = {0, 0, 0};
+  int cc_cps_log2[3] = {0, 0, 0};
+  int cc_xcd[3] = {0, 0, 0};
+  void *cc_F[3] = {nullptr, nullptr, nullptr};
+  bool tune_pending = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t cap = nullptr;
@@ -226,6 +234,82 @@ hipError_t launch_fused(tridAdiPlan_st *p, T *u, T *du, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Factor-reuse sweeps: chunk length R in {8, 16, 32} with at most 16 chunks
+// per system (work-groups of 64 systems x cps chunks).
+bool cc_rows_ok(int N, int R, int *cps_log2) {
+  if (R != 8 && R != 16 && R != 32) return false;
+  const long long ch = (N + R - 1) / R;
+  int cl = 0;
+  while ((1LL << cl) < ch) cl++;
+  if (cl > 4) return false;
+  *cps_log2 = cl;
+  return true;
+}
+
+// Builds the factor table of direction dir for chunk length R into *F
+// (allocated here; waits for the setup kernel).
+template <typename T>
+tridStatus_t cc_build(tridAdiPlan_st *p, int dir, int R, void **F, int *cps_log2) {
+  const int N = p->desc[dir].N;
+  if (!cc_rows_ok(N, R, cps_log2)) return fail(TRID_STATUS_INVALID_VALUE, "cc layout");
+  const int cps = 1 << *cps_log2;
+  size_t nf = 0;
+  if (R == 8) nf = CCLayout<8>::nf(*cps_log2);
+  else if (R == 16) nf = CCLayout<16>::nf(*cps_log2);
+  else nf = CCLayout<32>::nf(*cps_log2);
+  tridStatus_t s = check_hip(hipMalloc(F, cps * nf * sizeof(T)), "hipMalloc");
+  if (s) return s;
+  const CoefConst<T> cf = coefs<T>(p);
+  const int threads = std::max(cps, kWave);
+  hipStream_t st = current_stream();
+  if (R == 8)
+    hipLaunchKernelGGL((k_cc_setup<T, 8>), dim3(1), dim3(threads), 0, st, cf, N, *cps_log2,
+                       static_cast<T *>(*F));
+  else if (R == 16)
+    hipLaunchKernelGGL((k_cc_setup<T, 16>), dim3(1), dim3(threads), 0, st, cf, N, *cps_log2,
+                       static_cast<T *>(*F));
+  else
+    hipLaunchKernelGGL((k_cc_setup<T, 32>), dim3(1), dim3(threads), 0, st, cf, N, *cps_log2,
+                       static_cast<T *>(*F));
+  s = check_hip(hipGetLastError(), "factor setup launch");
+  if (!s) s = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+  if (s) {
+    (void)hipFree(*F);
+    *F = nullptr;
+  }
+  return s;
+}
+
+template <typename T, int INC>
+hipError_t launch_cc_t(int R, int cps_log2, int xcd, const void *F, const SysDesc &D, int idx32,
+                       T *d, T *u, hipStream_t st) {
+  const int cps = 1 << cps_log2;
+  const unsigned blocks = (unsigned)((D.nsys + kWave - 1) / kWave);
+  const size_t lds = 2 * (size_t)kWave * cps * sizeof(T);
+  const dim3 grid(blocks), block(kWave * cps);
+  const T *f = static_cast<const T *>(F);
+#define TRID_CC(RR, II)                                                                     \
+  hipLaunchKernelGGL((k_cc_sweep<T, INC, RR, II>), grid, block, lds, st, f, d, u, D, cps_log2, \
+                     xcd)
+  if (idx32) {
+    if (R == 8) TRID_CC(8, uint32_t);
+    else if (R == 16) TRID_CC(16, uint32_t);
+    else TRID_CC(32, uint32_t);
+  } else {
+    if (R == 8) TRID_CC(8, long long);
+    else if (R == 16) TRID_CC(16, long long);
+    else TRID_CC(32, long long);
+  }
+#undef TRID_CC
+  return hipGetLastError();
+}
+
+template <typename T, int INC>
+hipError_t launch_cc(const tridAdiPlan_st *p, int dir, T *d, T *u, hipStream_t st) {
+  return launch_cc_t<T, INC>(p->cc_R[dir], p->cc_cps_log2[dir], p->cc_xcd[dir], p->cc_F[dir],
+                             p->desc[dir], p->cfg[dir].idx32, d, u, st);
+}
+
 // Stage 4 = fused RHS + x sweep (plans with fused_R > 0 only).
 template <typename T>
 tridStatus_t stage_t(tridAdiPlan_st *p, T *u, T *du, int stage, hipStream_t st) {
@@ -242,9 +326,11 @@ tridStatus_t stage_t(tridAdiPlan_st *p, T *u, T *du, int stage, hipStream_t st) 
     const PartCfg &c = aligned16(du) ? p->cfg[0] : p->cfg_x_unaligned;
     e = launch_part<T, 0>(c, cf, du, du, p->desc[0], st);
   } else if (stage == 2) {
-    e = launch_part<T, 0>(p->cfg[1], cf, du, du, p->desc[1], st);
+    e = p->cc_R[1] ? launch_cc<T, 0>(p, 1, du, du, st)
+                   : launch_part<T, 0>(p->cfg[1], cf, du, du, p->desc[1], st);
   } else {
-    e = launch_part<T, 1>(p->cfg[2], cf, du, u, p->desc[2], st);
+    e = p->cc_R[2] ? launch_cc<T, 1>(p, 2, du, u, st)
+                   : launch_part<T, 1>(p->cfg[2], cf, du, u, p->desc[2], st);
   }
   return check_hip(e, "ADI kernel launch");
 }
@@ -296,17 +382,19 @@ void set_fused(tridAdiPlan_st *p, int R) {
 // and keeps the fastest.  Data values do not change
 // the kernels' work, so zero-filled scratch is enough.
 template <typename T>
-void tune_plan_t(tridAdiPlan_st *p) {
-  const size_t bytes = (size_t)p->nx * p->ny * p->nz * sizeof(T);
-  T *u = nullptr, *du = nullptr;
-  if (hipMalloc(&u, bytes) != hipSuccess) return;
-  if (hipMalloc(&du, bytes) != hipSuccess) {
-    (void)hipFree(u);
+void tune_plan_t(tridAdiPlan_st *p, T *u, T *du) {
+  // the candidates run on the caller's own arrays (the speed of large
+  // power-of-two strides depends on the actual addresses); u and du are
+  // saved first and restored at the end
+  const size_t npts = (size_t)p->nx * p->ny * p->nz, bytes = npts * sizeof(T);
+  T *save = nullptr;
+  if (hipMalloc(&save, 2 * bytes) != hipSuccess) {
+    (void)hipGetLastError();
     return;
   }
-  (void)hipMemset(u, 0, bytes);
-  (void)hipMemset(du, 0, bytes);
   hipStream_t st = current_stream();
+  (void)hipMemcpyAsync(save, u, bytes, hipMemcpyDeviceToDevice, st);
+  (void)hipMemcpyAsync(save + npts, du, bytes, hipMemcpyDeviceToDevice, st);
   const CoefConst<T> cf = coefs<T>(p);
   const int n[3] = {p->nx, p->ny, p->nz};
   for (int dir = 0; dir < 3; dir++) {
@@ -318,12 +406,51 @@ void tune_plan_t(tridAdiPlan_st *p) {
       c.idx32 = p->cfg[dir].idx32;
       cands.push_back(c);
     }
+    // y and z: also the factor-reuse kernel at every chunk length that fits
+    std::vector<int> ccR, ccL;
+    std::vector<void *> ccF;
+    if (dir > 0) {
+      for (int R : {8, 16, 32}) {
+        int cl = 0;
+        void *F = nullptr;
+        if (!cc_rows_ok(n[dir], R, &cl)) continue;
+        if (cc_build<T>(p, dir, R, &F, &cl) != TRID_STATUS_SUCCESS) continue;
+        ccR.push_back(R);
+        ccL.push_back(cl);
+        ccF.push_back(F);
+      }
+    }
+    const int G = (int)cands.size();
+    // candidate G + 2j + x: factor-reuse table j, XCD-contiguous order x
     auto run = [&](int i) {
+      if (i >= G) {
+        const int j = (i - G) >> 1, x = (i - G) & 1;
+        return dir == 2 ? launch_cc_t<T, 1>(ccR[j], ccL[j], x, ccF[j], p->desc[dir],
+                                            p->cfg[dir].idx32, du, u, st)
+                        : launch_cc_t<T, 0>(ccR[j], ccL[j], x, ccF[j], p->desc[dir],
+                                            p->cfg[dir].idx32, du, du, st);
+      }
       return dir == 2 ? launch_part<T, 1>(cands[i], cf, du, u, p->desc[dir], st)
                       : launch_part<T, 0>(cands[i], cf, du, du, p->desc[dir], st);
     };
-    const int i = pick_fastest((int)cands.size(), [](int) { return true; }, run, st);
-    if (i >= 0) p->cfg[dir] = cands[i];
+    const int i = pick_fastest(G + 2 * (int)ccR.size(), [](int) { return true; }, run, st);
+    if (i >= 0 && i < G) p->cfg[dir] = cands[i];
+    (void)hipStreamSynchronize(st);
+    if (i >= G) {
+      const int j = (i - G) >> 1;
+      if (p->cc_F[dir]) (void)hipFree(p->cc_F[dir]);
+      p->cc_R[dir] = ccR[j];
+      p->cc_cps_log2[dir] = ccL[j];
+      p->cc_xcd[dir] = (i - G) & 1;
+      p->cc_F[dir] = ccF[j];
+      ccF[j] = nullptr;
+    } else if (i >= 0 && p->cc_F[dir]) {
+      (void)hipFree(p->cc_F[dir]);
+      p->cc_F[dir] = nullptr;
+      p->cc_R[dir] = 0;
+    }
+    for (void *F : ccF)
+      if (F) (void)hipFree(F);
   }
   if (p->fused_R) {
     std::vector<int> rows;
@@ -336,17 +463,37 @@ void tune_plan_t(tridAdiPlan_st *p) {
     const int i = pick_fastest((int)rows.size(), [](int) { return true; }, run, st);
     set_fused(p, i >= 0 ? rows[i] : p->fused_R);
   }
+  (void)hipMemcpyAsync(u, save, bytes, hipMemcpyDeviceToDevice, st);
+  (void)hipMemcpyAsync(du, save + npts, bytes, hipMemcpyDeviceToDevice, st);
   (void)hipStreamSynchronize(st);
-  (void)hipFree(u);
-  (void)hipFree(du);
+  (void)hipFree(save);
   (void)hipGetLastError();
 }
 
-void tune_plan(tridAdiPlan_st *p) {
+std::string plan_text(const tridAdiPlan_st *p) {
+  const size_t ts = p->dbl ? 8 : 4;
+  auto sweep_name = [&](int dir) {
+    if (!p->cc_R[dir]) return cfg_name(p->cfg[dir], ts);
+    return "cc<R" + std::to_string(p->cc_R[dir]) + ",cps" +
+           std::to_string(1 << p->cc_cps_log2[dir]) + (p->cc_xcd[dir] ? ",xcd>" : ">");
+  };
+  return std::string(p->fused_R ? "rhs+x(fused,R" + std::to_string(p->fused_R) + ")<"
+                                 : "rhs<") +
+         (p->dbl ? "double" : "float") + "> x:" + cfg_name(p->cfg[0], ts) +
+         " y:" + sweep_name(1) + " z:" + sweep_name(2);
+}
+
+// With autotuning on, a plan tunes at its first use, on that call's arrays
+// (never inside a graph capture: tridAdiGraphCapture tunes before it starts
+// capturing).
+void ensure_tuned(tridAdiPlan_st *p, void *u, void *du) {
+  if (!p->tune_pending) return;
+  p->tune_pending = false;
   if (p->dbl)
-    tune_plan_t<double>(p);
+    tune_plan_t<double>(p, static_cast<double *>(u), static_cast<double *>(du));
   else
-    tune_plan_t<float>(p);
+    tune_plan_t<float>(p, static_cast<float *>(u), static_cast<float *>(du));
+  p->text = plan_text(p);
 }
 
 tridStatus_t check_args(tridAdiPlan_t p, void *u, void *du) {
@@ -401,10 +548,25 @@ EXTERN_C tridStatus_t tridAdiPlanCreate(tridAdiPlan_t *plan, int nx, int ny,
         break;
       }
   }
-  if (autotune_mode() == 1) tune_plan(p);
-  p->text = std::string(p->fused_R ? "rhs+x(fused,R" + std::to_string(p->fused_R) + ")<" : "rhs<") +
-            (p->dbl ? "double" : "float") + "> x:" + cfg_name(p->cfg[0], ts) +
-            " y:" + cfg_name(p->cfg[1], ts) + " z:" + cfg_name(p->cfg[2], ts);
+  // y and z sweeps: factor reuse by default when a chunk layout fits
+  for (int dir = 1; dir < 3; dir++) {
+    for (int R : {16, 8, 32}) {
+      int cl = 0;
+      if (!cc_rows_ok(n[dir], R, &cl)) continue;
+      void *F = nullptr;
+      const tridStatus_t r = p->dbl ? cc_build<double>(p, dir, R, &F, &cl)
+                                    : cc_build<float>(p, dir, R, &F, &cl);
+      if (r == TRID_STATUS_SUCCESS) {
+        p->cc_R[dir] = R;
+        p->cc_cps_log2[dir] = cl;
+        p->cc_F[dir] = F;
+      }
+      break;
+    }
+  }
+  clear_error();
+  p->tune_pending = autotune_mode() == 1;
+  p->text = plan_text(p);
   *plan = p;
   return TRID_STATUS_SUCCESS;
 }
@@ -414,6 +576,8 @@ EXTERN_C tridStatus_t tridAdiPlanDestroy(tridAdiPlan_t p) {
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->cap) (void)hipStreamDestroy(p->cap);
+  for (void *F : p->cc_F)
+    if (F) (void)hipFree(F);
   delete p;
   return TRID_STATUS_SUCCESS;
 }
@@ -424,6 +588,7 @@ EXTERN_C tridStatus_t tridAdiStep(tridAdiPlan_t p, void *u, void *du, int nsteps
   tridStatus_t s = check_args(p, u, du);
   if (s) return s;
   if (nsteps < 0) return fail(TRID_STATUS_INVALID_VALUE, "nsteps < 0");
+  ensure_tuned(p, u, du);
   hipStream_t st = current_stream();
   s = steps_any(p, u, du, nsteps, st);
   if (s) return s;
@@ -436,6 +601,7 @@ EXTERN_C tridStatus_t tridAdiStage(tridAdiPlan_t p, void *u, void *du, int stage
   tridStatus_t s = check_args(p, u, du);
   if (s) return s;
   if (stage < 0 || stage > 4) return fail(TRID_STATUS_INVALID_VALUE, "stage must be 0..4");
+  ensure_tuned(p, u, du);
   if (stage == 4 && !use_fused(p, u, du))
     return fail(TRID_STATUS_INVALID_VALUE, "this plan/layout has no fused RHS + x stage");
   return stage_any(p, u, du, stage, current_stream());
@@ -447,6 +613,7 @@ EXTERN_C tridStatus_t tridAdiGraphCapture(tridAdiPlan_t p, void *u, void *du,
   tridStatus_t s = check_args(p, u, du);
   if (s) return s;
   if (nsteps < 1) return fail(TRID_STATUS_INVALID_VALUE, "nsteps must be >= 1");
+  ensure_tuned(p, u, du);
   if (!p->cap) {
     s = check_hip(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking),
                   "hipStreamCreate");

@@ -1,0 +1,271 @@
+// trid_cc.hpp -- partitioned Thomas with precomputed factors ("factor
+// reuse") for batches in which every interior system has the same constant
+// tridiagonal matrix: the ADI y and z sweeps (constant a, b, c on rows
+// interior to the grid, identity rows elsewhere).
+//
+// In the partitioned solve of trid_part.hpp every multiplier that touches d
+// -- the pivot reciprocals and the sub-diagonal entries of the chunk sweep,
+// the backward-substitution and expansion multipliers, the first/last row
+// coefficients and the (a, c, 1/diagonal) of every reduced-system PCR level
+// -- depends only on the matrix, the chunk index and the chunk layout.  For a
+// constant-coefficient matrix they are the same for every interior line, so
+// k_cc_setup computes them once (per plan) with the very same recurrences,
+// and k_cc_sweep only runs the d recurrences: no reciprocals, and one value
+// (d) per reduced-system exchange instead of three (a, c, d).  The reference
+// keeps precomputed factors for repeated matrices in the same spirit
+// (trid_warp_setup / trid_warp_solve, scalar/src/cuda/trid_thomaspcr_small.h:
+// 508-605); this is an independent design for the chunked wave64 scheme.
+//
+// Layout of k_cc_sweep: systems in consecutive lanes (coalesced rows, as
+// ORDER 1 of trid_part.hpp) with exactly 64 systems per work-group, so wave w
+// holds chunk w of 64 systems and the chunk's factors are wave-uniform
+// (loaded once per wave through the scalar cache).  Lines that are not
+// interior to the grid are identity systems: their lanes store d unchanged.
+#pragma once
+
+#include "trid_part.hpp"
+
+namespace tridhip {
+
+// Per-chunk factor record (T values) for chunk length R.
+template <int R>
+struct CCLayout {
+  static_assert(R >= 4, "factor layout needs R >= 4");
+  static constexpr int R1 = 0;              // 1/b_1
+  static constexpr int A = 1;               // a_i, i = 2..R-1
+  static constexpr int RI = A + (R - 2);    // 1/(b_i - a_i c'_{i-1}), i = 2..R-1
+  static constexpr int CB = RI + (R - 2);   // c'_i of the backward pass, i = 1..R-3
+  static constexpr int C0 = CB + (R - 3);   // c_0
+  static constexpr int R0 = C0 + 1;         // 1/(first row's diagonal)
+  static constexpr int EA = R0 + 1;         // expansion a_i, i = 1..R-2
+  static constexpr int EC = EA + (R - 2);   // expansion c_i, i = 1..R-2
+  static constexpr int FA = EC + (R - 2);   // first row a, c
+  static constexpr int FC = FA + 1;
+  static constexpr int LA = FC + 1;         // last row a, c and 1/diagonal
+  static constexpr int LC = LA + 1;         //   after eliminating the first
+  static constexpr int INV = LC + 1;        //   unknown of the next chunk
+  static constexpr int PCR = INV + 1;       // per PCR level: r.a, r.c, 1/bn
+  __host__ __device__ static constexpr int nf(int levels) { return PCR + 3 * levels; }
+};
+
+// Factors of the interior-line matrix of a constant-coefficient batch (rows
+// interior to the grid when 1 <= row <= N-2: a, b, c = cf; identity rows
+// otherwise), for chunk length R and cps = 2^cps_log2 chunks.  One
+// work-group of max(cps, 64) threads; thread k writes chunk k's record at
+// F + k * nf.  The recurrences are those of chunk_reduce, eliminate_first
+// and pcr_update, applied to the coefficients only.
+template <typename T, int R>
+__global__ void __launch_bounds__(1024)
+    k_cc_setup(CoefConst<T> cf, int N, int cps_log2, T *__restrict__ F) {
+  using Ly = CCLayout<R>;
+  __shared__ T sa[1024], sc[1024];
+  const int cps = 1 << cps_log2;
+  const int nf = Ly::nf(cps_log2);
+  const int k = threadIdx.x;
+  const bool on = k < cps;
+  T *out = F + (long long)(on ? k : 0) * nf;
+  auto W = [&](int i, T v) {
+    if (on) out[i] = v;
+  };
+  T a[R], b[R], c[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int row = k * R + r;
+    const bool in = row >= 1 && row + 2 <= N;
+    a[r] = in ? cf.a : T(0);
+    b[r] = in ? cf.b : T(1);
+    c[r] = in ? cf.c : T(0);
+  }
+  T ap, cp;
+  {
+    const T r = rcp(b[1]);
+    ap = a[1] * r;
+    cp = c[1] * r;
+    a[1] = ap;
+    c[1] = cp;
+    W(Ly::R1, r);
+  }
+#pragma unroll
+  for (int i = 2; i < R; i++) {
+    const T r = rcp(b[i] - a[i] * cp);
+    W(Ly::A + i - 2, a[i]);
+    W(Ly::RI + i - 2, r);
+    ap = -a[i] * ap * r;
+    cp = c[i] * r;
+    a[i] = ap;
+    c[i] = cp;
+  }
+  const T la = a[R - 1], lc = c[R - 1];
+#pragma unroll
+  for (int i = R - 3; i >= 1; i--) {
+    W(Ly::CB + i - 1, c[i]);
+    a[i] = a[i] - c[i] * a[i + 1];
+    c[i] = -c[i] * c[i + 1];
+  }
+  const T bf = b[0] - c[0] * a[1];
+  const T cff = -c[0] * c[1];
+  W(Ly::C0, c[0]);
+  const T r0 = rcp(bf);
+  W(Ly::R0, r0);
+  const T fa = a[0] * r0, fc = cff * r0;
+#pragma unroll
+  for (int i = 1; i < R - 1; i++) {
+    W(Ly::EA + i - 1, a[i]);
+    W(Ly::EC + i - 1, c[i]);
+  }
+  W(Ly::FA, fa);
+  W(Ly::FC, fc);
+  // reduced system: eliminate the next chunk's first unknown
+  if (on) {
+    sa[k] = fa;
+    sc[k] = fc;
+  }
+  __syncthreads();
+  const T fna = k + 1 < cps ? sa[k + 1] : T(0);
+  const T fnc = k + 1 < cps ? sc[k + 1] : T(0);
+  T inv = rcp(T(1) - la * fc - lc * fna);
+  W(Ly::LA, la);
+  W(Ly::LC, lc);
+  W(Ly::INV, inv);
+  T ra = -la * fa * inv, rc = -lc * fnc * inv;
+  int j = 0;
+  for (int s = 1; s < cps; s <<= 1, j++) {
+    __syncthreads();
+    if (on) {
+      sa[k] = ra;
+      sc[k] = rc;
+    }
+    __syncthreads();
+    const T ma = k >= s && on ? sa[k - s] : T(0), mc = k >= s && on ? sc[k - s] : T(0);
+    const T pa = k + s < cps ? sa[k + s] : T(0), pc = k + s < cps ? sc[k + s] : T(0);
+    inv = rcp(T(1) - ra * mc - rc * pa);
+    W(Ly::PCR + 3 * j, ra);
+    W(Ly::PCR + 3 * j + 1, rc);
+    W(Ly::PCR + 3 * j + 2, inv);
+    ra = -ra * ma * inv;
+    rc = -rc * pc * inv;
+  }
+}
+
+// The solve with precomputed factors F (k_cc_setup of the same N, R, cps).
+// blockDim = 64 * cps; system q = 64 * block + lane; wave w = chunk w.
+// Non-Inc: d <- A^{-1} d; Inc: u <- u + A^{-1} d.
+template <typename T, int INC, int R, typename I>
+__global__ void __launch_bounds__(1024)
+    k_cc_sweep(const T *__restrict__ F, T *__restrict__ Dd, T *__restrict__ U, SysDesc D,
+               int cps_log2, int xcd) {
+  using Ly = CCLayout<R>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *sm = reinterpret_cast<T *>(smem_raw);
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int cps = 1 << cps_log2;
+  const int k = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform chunk index
+  const T *__restrict__ Fk = F + k * Ly::nf(cps_log2);
+  // xcd: every XCD takes one contiguous range of systems (as k_part)
+  long long block = blockIdx.x;
+  if (xcd) {
+    const unsigned nb8 = gridDim.x & ~7u;
+    if (blockIdx.x < nb8) block = (long long)(blockIdx.x & 7u) * (nb8 >> 3) + (blockIdx.x >> 3);
+  }
+  const long long q = block * kWave + lane;
+  const bool active = q < D.nsys;
+  const int N = D.N;
+  I off[5];
+  bool line_in = false;
+  if (active) {
+    sys_offsets<I>(D, q, off, &line_in);
+  } else {
+#pragma unroll
+    for (int t = 0; t < 5; t++) off[t] = 0;
+  }
+  const int row0 = k * R;
+  const I s3 = (I)D.sstr[3], s4 = (I)D.sstr[4];
+  T d[R];
+  {
+    I od = off[3] + (I)row0 * s3;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      d[r] = active && row0 + r < N ? ld_nt(at(Dd, od)) : T(0);
+      od += s3;
+    }
+  }
+  // chunk sweep (d only)
+  {
+    T dp = d[1] * Fk[Ly::R1];
+    d[1] = dp;
+#pragma unroll
+    for (int i = 2; i < R; i++) {
+      dp = (d[i] - Fk[Ly::A + i - 2] * dp) * Fk[Ly::RI + i - 2];
+      d[i] = dp;
+    }
+  }
+  const T ld = d[R - 1];
+#pragma unroll
+  for (int i = R - 3; i >= 1; i--) d[i] = d[i] - Fk[Ly::CB + i - 1] * d[i + 1];
+  const T fd = (d[0] - Fk[Ly::C0] * d[1]) * Fk[Ly::R0];
+  // reduced system, d only, through LDS (ping-pong buffers of 64 * cps)
+  const int n = cps * kWave;
+  const int me = k * kWave + lane;
+  int par = 0;
+  sm[me] = fd;
+  __syncthreads();
+  const T fnd = k + 1 < cps ? sm[me + kWave] : T(0);
+  par = 1;
+  T rd = (ld - Fk[Ly::LA] * fd - Fk[Ly::LC] * fnd) * Fk[Ly::INV];
+  int j = 0;
+  for (int s = 1; s < cps; s <<= 1, j++) {
+    T *b = sm + par * n;
+    b[me] = rd;
+    __syncthreads();
+    const T md = k >= s ? b[me - s * kWave] : T(0);
+    const T pd = k + s < cps ? b[me + s * kWave] : T(0);
+    par ^= 1;
+    rd = (rd - Fk[Ly::PCR + 3 * j] * md - Fk[Ly::PCR + 3 * j + 1] * pd) *
+         Fk[Ly::PCR + 3 * j + 2];
+  }
+  const T xl = rd;
+  T xf;
+  {
+    T *b = sm + par * n;
+    b[me] = xl;
+    __syncthreads();
+    const T xm = k > 0 ? b[me - kWave] : T(0);
+    xf = fd - Fk[Ly::FA] * xm - Fk[Ly::FC] * xl;
+  }
+  T y[R];
+  y[0] = xf;
+  y[R - 1] = xl;
+#pragma unroll
+  for (int i = 1; i < R - 1; i++) y[i] = d[i] - Fk[Ly::EA + i - 1] * xf - Fk[Ly::EC + i - 1] * xl;
+  if (!active) return;
+  if (!line_in) {
+    // identity system: the solution is d itself (not yet overwritten)
+    I od = off[3] + (I)row0 * s3;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (row0 + r < N) y[r] = *at(Dd, od);
+      od += s3;
+    }
+  }
+  if (INC) {
+    I ou = off[4] + (I)row0 * s4;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (row0 + r < N) {
+        T *pu = at(U, ou);
+        st_nt(pu, ld_nt(pu) + y[r]);
+      }
+      ou += s4;
+    }
+  } else {
+    I od = off[3] + (I)row0 * s3;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (row0 + r < N) st_nt(at(Dd, od), y[r]);
+      od += s3;
+    }
+  }
+}
+
+}  // namespace tridhip

@@ -2,9 +2,9 @@
 // (include/trid_ragged.h).
 //
 // Every system is solved with the partitioned Thomas scheme of
-// trid_part.hpp (chunks of R rows per lane, reduced system by PCR), with the
-// number of chunks per system (cps) chosen per length bin.  A work item is
-// one work-group of 256 threads processing 256/cps systems of one bin.
+// trid_part.hpp (chunks of R = 8 rows per lane, reduced system by PCR), with the number of chunks per system (cps,
+// 1..512) chosen per length bin.  A work item is one work-group of 512
+// threads processing 512/cps systems of one bin.
 // Loads are staged per system group through LDS: the cps lanes of a group
 // read the system's contiguous span with consecutive lanes on consecutive
 // elements, then each lane takes its chunk from LDS (the same transpose as
@@ -24,14 +24,22 @@
 
 namespace tridhip {
 
-constexpr int kRagThreads = 256;
+constexpr int kRagThreads = 512;
 constexpr int kRagMaxLen = 4096;  // longest system of the persistent kernel
 
+// Chunk length of the persistent kernel: every bin uses it, only the chunks
+// per system (1..512) change, so one kernel serves all bins at the register
+// cost of one chunk length (512 chunks of 8 rows cover 4096 rows).
+template <typename T>
+constexpr int rag_rows() {
+  return 8;
+}
+
 struct RagItem {
-  int R;         // chunk length (8 or 16)
   int cps_log2;  // chunks per system
   int first;     // first entry of the item in the sorted system list
-  int count;     // systems in this item (<= 256 / cps)
+  int count;     // systems in this item (<= kRagThreads / cps)
+  int pad;
 };
 
 template <typename T>
@@ -46,10 +54,9 @@ struct RagArgs {
   unsigned *counter;  // [0] next item, [1] finished work-groups
 };
 
-// Group-staged access: the cps lanes of a system group move the system's
-// span [off, off+L) between global memory and the group's LDS region with
-// consecutive lanes on consecutive elements.
-template <int R>
+// Barrier over the lanes of one system group: a wave barrier when the
+// group fits in one wave (cps <= 64; LDS operations of a wave execute in
+// order), the work-group barrier otherwise.
 __device__ __forceinline__ void rag_sync(int cps) {
   if (cps > kWave)
     __syncthreads();
@@ -57,27 +64,41 @@ __device__ __forceinline__ void rag_sync(int cps) {
     __builtin_amdgcn_wave_barrier();
 }
 
+// Group-staged access: the cps lanes of a system group read the system's
+// span [off, off+L) with consecutive lanes on consecutive elements (element
+// r*cps + k in the lane's slot r), then every lane takes its own chunk
+// (elements k*R .. k*R+R-1) from the group's LDS region.
 template <typename T, int R>
-__device__ __forceinline__ void rag_load(const T *__restrict__ X, long long off,
-                                         int L, int k, int cps, T *st,
-                                         T (&x)[R], T pad) {
+__device__ __forceinline__ void rag_issue(const T *__restrict__ X, long long off, int L, int k,
+                                          int cps, T (&g)[R]) {
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const int e = r * cps + k;
-    if (e < L) st[e] = X[off + e];
+    g[r] = e < L ? X[off + e] : T(0);
   }
-  rag_sync<R>(cps);
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void rag_put(const T (&g)[R], int L, int k, int cps, T *st) {
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int e = r * cps + k;
+    if (e < L) st[e] = g[r];
+  }
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void rag_take(const T *st, int L, int k, T (&x)[R], T pad) {
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const int e = k * R + r;
     x[r] = e < L ? st[e] : pad;
   }
-  rag_sync<R>(cps);
 }
 
-template <typename T, int R>
-__device__ __forceinline__ void rag_item(const RagArgs<T> &g, const RagItem &it,
-                                         T *smem) {
+template <typename T>
+__device__ __forceinline__ void rag_item(const RagArgs<T> &g, const RagItem &it, T *smem) {
+  constexpr int R = rag_rows<T>();
   const int tid = threadIdx.x;
   const int cps = 1 << it.cps_log2;
   const int k = tid & (cps - 1);
@@ -90,12 +111,27 @@ __device__ __forceinline__ void rag_item(const RagArgs<T> &g, const RagItem &it,
     off = g.offsets[id];
     L = (int)(g.offsets[id + 1] - off);
   }
-  T *st = smem + slot * (cps * R);
+  // two staging buffers of kRagThreads*R elements; the group's region
+  T *st0 = smem + slot * (cps * R);
+  T *st1 = st0 + kRagThreads * R;
+  // all four arrays' global loads in flight before the first transpose
   T a[R], b[R], c[R], d[R];
-  rag_load<T, R>(g.a, off, L, k, cps, st, a, T(0));
-  rag_load<T, R>(g.b, off, L, k, cps, st, b, T(1));
-  rag_load<T, R>(g.c, off, L, k, cps, st, c, T(0));
-  rag_load<T, R>(g.d, off, L, k, cps, st, d, T(0));
+  rag_issue<T, R>(g.a, off, L, k, cps, a);
+  rag_issue<T, R>(g.b, off, L, k, cps, b);
+  rag_issue<T, R>(g.c, off, L, k, cps, c);
+  rag_issue<T, R>(g.d, off, L, k, cps, d);
+  rag_put<T, R>(a, L, k, cps, st0);
+  rag_put<T, R>(b, L, k, cps, st1);
+  rag_sync(cps);
+  rag_take<T, R>(st0, L, k, a, T(0));
+  rag_take<T, R>(st1, L, k, b, T(1));
+  rag_sync(cps);
+  rag_put<T, R>(c, L, k, cps, st0);
+  rag_put<T, R>(d, L, k, cps, st1);
+  rag_sync(cps);
+  rag_take<T, R>(st0, L, k, c, T(0));
+  rag_take<T, R>(st1, L, k, d, T(0));
+  rag_sync(cps);
   const int row0 = k * R;
   if (row0 == 0) a[0] = T(0);
 #pragma unroll
@@ -108,8 +144,8 @@ __device__ __forceinline__ void rag_item(const RagArgs<T> &g, const RagItem &it,
   if (cps <= kWave) {
     reduced_wave<T>(f, l, k, cps, tid & (kWave - 1), xf, xl);
   } else {
-    // the PCR exchange buffers reuse the staging region (6 values per
-    // thread <= R per thread); the loads above are complete (barrier)
+    // the PCR exchange buffers (6 values per thread) reuse the staging
+    // region; the loads above are complete (barrier)
     reduced_lds<T>(f, l, k, cps, slot, kRagThreads >> it.cps_log2, smem, xf, xl);
     __syncthreads();
   }
@@ -118,39 +154,37 @@ __device__ __forceinline__ void rag_item(const RagArgs<T> &g, const RagItem &it,
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const int e = row0 + r;
-    if (e < L) st[e] = y[r];
+    if (e < L) st0[e] = y[r];
   }
-  rag_sync<R>(cps);
+  rag_sync(cps);
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const int e = r * cps + k;
     if (e < L) {
       if (g.inc)
-        g.u[off + e] += st[e];
+        g.u[off + e] += st0[e];
       else
-        g.d[off + e] = st[e];
+        g.d[off + e] = st0[e];
     }
   }
-  // the next item reuses the staging region
-  __syncthreads();
 }
 
+// Persistent work-groups take items from an atomic counter; the next
+// item's index is fetched while the current item is processed.
 template <typename T>
 __global__ void __launch_bounds__(kRagThreads) k_ragged(RagArgs<T> g) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *smem = reinterpret_cast<T *>(smem_raw);
-  __shared__ int s_item;
-  for (;;) {
-    if (threadIdx.x == 0) s_item = (int)atomicAdd(g.counter, 1u);
-    __syncthreads();
-    const int i = s_item;
-    __syncthreads();
-    if (i >= g.nitems) break;
-    const RagItem it = g.items[i];
-    if (it.R == 8)
-      rag_item<T, 8>(g, it, smem);
-    else
-      rag_item<T, 16>(g, it, smem);
+  __shared__ int s_next;
+  if (threadIdx.x == 0) s_next = (int)atomicAdd(g.counter, 1u);
+  __syncthreads();
+  int i = s_next;
+  while (i < g.nitems) {
+    __syncthreads();  // every lane has read s_next
+    if (threadIdx.x == 0) s_next = (int)atomicAdd(g.counter, 1u);
+    rag_item<T>(g, g.items[i], smem);
+    __syncthreads();  // s_next written; the next item reuses the staging region
+    i = s_next;
   }
   // the last work-group to finish resets the counters for the next launch
   if (threadIdx.x == 0) {
@@ -294,15 +328,15 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
       free_plan(p);
       return fail(TRID_STATUS_INVALID_VALUE, "offsets must be non-decreasing");
     }
-  // bins: R = 8 for L <= 64*8 (cps 1..64), R = 16 for L <= 4096 (cps 64..256)
-  const int Rsmall = p->dbl ? 8 : 16;
-  auto bin_of = [&](long long L, int &R, int &cl) {
-    R = L <= 64LL * Rsmall ? Rsmall : 16;
-    long long ch = (L + R - 1) / R;
-    cl = 0;
+  // bins by chunks per system (cps = 2^cl >= L / R), one chunk length R
+  const int R = p->dbl ? rag_rows<double>() : rag_rows<float>();
+  auto bin_of = [&](long long L) {
+    const long long ch = (L + R - 1) / R;
+    int cl = 0;
     while ((1LL << cl) < ch) cl++;
+    return cl;
   };
-  std::vector<std::vector<int>> bins(2 * 16);
+  std::vector<std::vector<int>> bins(16);
   std::vector<int> longs;
   std::vector<long long> long_off;
   long long lws = 0;
@@ -315,33 +349,28 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
       lws += 2 * L;
       continue;
     }
-    int R, cl;
-    bin_of(L, R, cl);
-    bins[(R == 16 ? 16 : 0) + cl].push_back((int)k);
+    bins[bin_of(L)].push_back((int)k);
   }
-  // items, most expensive first: larger R*cps first
+  // items, most expensive first (more chunks per system first)
   std::vector<int> sys;
   std::vector<RagItem> items;
   std::string desc;
-  for (int Rsel : {16, 8}) {
-    for (int cl = 15; cl >= 0; cl--) {
-      std::vector<int> &v = bins[(Rsel == 16 ? 16 : 0) + cl];
-      if (v.empty()) continue;
-      if (Rsel != Rsmall && Rsel != 16) continue;
-      const int per = std::max(1, kRagThreads >> cl);
-      for (size_t i = 0; i < v.size(); i += per) {
-        RagItem it;
-        it.R = Rsel;
-        it.cps_log2 = cl;
-        it.first = (int)sys.size() + (int)i;
-        it.count = (int)std::min<size_t>(per, v.size() - i);
-        items.push_back(it);
-      }
-      sys.insert(sys.end(), v.begin(), v.end());
-      char buf[96];
-      snprintf(buf, sizeof buf, "R%d/cps%d:%zu ", Rsel, 1 << cl, v.size());
-      desc += buf;
+  for (int cl = 15; cl >= 0; cl--) {
+    std::vector<int> &v = bins[cl];
+    if (v.empty()) continue;
+    const int per = std::max(1, kRagThreads >> cl);
+    for (size_t i = 0; i < v.size(); i += per) {
+      RagItem it;
+      it.cps_log2 = cl;
+      it.first = (int)sys.size() + (int)i;
+      it.count = (int)std::min<size_t>(per, v.size() - i);
+      it.pad = 0;
+      items.push_back(it);
     }
+    sys.insert(sys.end(), v.begin(), v.end());
+    char buf[96];
+    snprintf(buf, sizeof buf, "R%d/cps%d:%zu ", R, 1 << cl, v.size());
+    desc += buf;
   }
   if (!longs.empty()) desc += "long(>4096):" + std::to_string(longs.size());
   p->text = desc;
@@ -349,7 +378,7 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
   p->nlong = (int)longs.size();
   p->long_ws = lws;
   const size_t ts = p->dbl ? 8 : 4;
-  p->lds = (size_t)kRagThreads * 16 * ts;
+  p->lds = 2 * (size_t)kRagThreads * R * ts;  // two staging buffers
   auto up = [&](void **dst, const void *src, size_t bytes) -> tridStatus_t {
     if (bytes == 0) return TRID_STATUS_SUCCESS;
     tridStatus_t r = check_hip(hipMalloc(dst, bytes), "hipMalloc");
@@ -368,11 +397,19 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
     free_plan(p);
     return s;
   }
-  // persistent grid: enough work-groups to fill every CU several times
+  // persistent grid: as many work-groups as fit on the device at once
   int dev = 0, cus = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  p->grid = std::max(1, std::min(p->nitems, cus * 4));
+  int per_cu = 0;
+  if (p->dbl)
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ragged<double>, kRagThreads,
+                                                       p->lds);
+  else
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ragged<float>, kRagThreads,
+                                                       p->lds);
+  per_cu = std::max(1, per_cu);
+  p->grid = std::max(1, std::min(p->nitems, cus * per_cu));
   *plan = p;
   return TRID_STATUS_SUCCESS;
 }

@@ -142,23 +142,23 @@ PartCfg autotune(int order, int vec, int idx32, bool dense, const T *a,
   if (stage_ok(best, dense, D.nsys, D.N)) best.vec = 2;
   best.idx32 = idx32;
   if (cands.size() < 2) return best;
-  // scratch copy of the written array so that tuning leaves the caller's
-  // data untouched
-  T *scratch = static_cast<T *>(workspace((size_t)d_extent * sizeof(T)));
-  if (!scratch) return best;
-  T *src = INC ? u : d;
-  T *dd = INC ? d : scratch;
-  T *uu = INC ? scratch : u;
-  // every candidate starts from a fresh copy of the written array
+  // the candidates run on the caller's own arrays (memory-channel effects of
+  // large power-of-two strides depend on the actual addresses); the written
+  // array is saved first, restored before every candidate and at the end
+  T *saved = static_cast<T *>(workspace((size_t)d_extent * sizeof(T)));
+  if (!saved) return best;
+  T *dst = INC ? u : d;
+  const size_t bytes = (size_t)d_extent * sizeof(T);
+  if (hipMemcpyAsync(saved, dst, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess) return best;
   auto prep = [&](int) {
-    return hipMemcpyAsync(scratch, src, (size_t)d_extent * sizeof(T),
-                          hipMemcpyDeviceToDevice, st) == hipSuccess;
+    return hipMemcpyAsync(dst, saved, bytes, hipMemcpyDeviceToDevice, st) == hipSuccess;
   };
   auto run = [&](int i) {
-    return launch_part<T, INC>(cands[i], CoefArrays<T>{a, b, c}, dd, uu, D, st);
+    return launch_part<T, INC>(cands[i], CoefArrays<T>{a, b, c}, d, u, D, st);
   };
   const int i = pick_fastest((int)cands.size(), prep, run, st);
   if (i >= 0) best = cands[i];
+  (void)hipMemcpyAsync(dst, saved, bytes, hipMemcpyDeviceToDevice, st);
   return best;
 }
 
