@@ -26,7 +26,11 @@
 
 typedef struct tridAdiPlan_st *tridAdiPlan_t;
 
-/* is_double: 1 = FP64, 0 = FP32.  nx, ny, nz >= 1. */
+/* is_double: 1 = FP64, 0 = FP32.  1 <= nx, ny, nz <= 8192.  The y and z
+ * sweeps of lines of at most 512 rows use precomputed factors of the
+ * constant-coefficient matrix (computed here, once).  With autotuning on
+ * (tridSetAutotune), the plan times its kernel choices at its first use, on
+ * the arrays of that call, inside the time step, and keeps the fastest. */
 EXTERN_C tridStatus_t tridAdiPlanCreate(tridAdiPlan_t *plan, int nx, int ny,
                                         int nz, int is_double, double lambda,
                                         int mode);

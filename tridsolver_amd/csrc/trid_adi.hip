@@ -381,11 +381,62 @@ void set_fused(tridAdiPlan_st *p, int R) {
 // sweep, and the chunk length of the fused RHS + x kernel) on scratch grids
 // and keeps the fastest.  Data values do not change
 // the kernels' work, so zero-filled scratch is enough.
+// Times the candidates of one stage inside the time step: for every timed
+// step, prefix() runs the stages before it, the candidate runs between two
+// events and suffix() runs the stages after it, so the candidate sees the
+// caches as a real step leaves them.  `passes` interleaved rounds of one
+// warm-up step and `reps` timed steps; returns the candidate with the lowest
+// time of its best round (-1 if none ran).
+template <typename Prefix, typename Cand, typename Suffix>
+int pick_in_step(int ncand, Prefix prefix, Cand cand, Suffix suffix, hipStream_t st,
+                 int reps = 4, int passes = 2) {
+  std::vector<hipEvent_t> ev(2 * reps, nullptr);
+  for (auto &e : ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      for (auto &f : ev)
+        if (f) (void)hipEventDestroy(f);
+      (void)hipGetLastError();
+      return -1;
+    }
+  std::vector<float> best(ncand, 1e30f);
+  for (int pass = 0; pass < passes; pass++)
+    for (int i = 0; i < ncand; i++) {
+      prefix();
+      if (cand(i) != hipSuccess) {
+        (void)hipGetLastError();
+        continue;
+      }
+      suffix();
+      for (int r = 0; r < reps; r++) {
+        prefix();
+        (void)hipEventRecord(ev[2 * r], st);
+        (void)cand(i);
+        (void)hipEventRecord(ev[2 * r + 1], st);
+        suffix();
+      }
+      if (hipEventSynchronize(ev[2 * reps - 1]) != hipSuccess) break;
+      float tot = 0.f;
+      for (int r = 0; r < reps; r++) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]);
+        tot += ms;
+      }
+      best[i] = std::min(best[i], tot);
+    }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  (void)hipGetLastError();
+  int arg = -1;
+  for (int i = 0; i < ncand; i++)
+    if (best[i] < 1e29f && (arg < 0 || best[i] < best[arg])) arg = i;
+  return arg;
+}
+
+// Tunes the plan on the caller's own arrays (the speed of large power-of-two
+// strides depends on the actual addresses): the y sweep, then the z sweep,
+// then the chunk length of the fused RHS + x kernel, each timed inside the
+// time step.  u and du are saved first and restored at the end.
 template <typename T>
 void tune_plan_t(tridAdiPlan_st *p, T *u, T *du) {
-  // the candidates run on the caller's own arrays (the speed of large
-  // power-of-two strides depends on the actual addresses); u and du are
-  // saved first and restored at the end
   const size_t npts = (size_t)p->nx * p->ny * p->nz, bytes = npts * sizeof(T);
   T *save = nullptr;
   if (hipMalloc(&save, 2 * bytes) != hipSuccess) {
@@ -397,71 +448,106 @@ void tune_plan_t(tridAdiPlan_st *p, T *u, T *du) {
   (void)hipMemcpyAsync(save + npts, du, bytes, hipMemcpyDeviceToDevice, st);
   const CoefConst<T> cf = coefs<T>(p);
   const int n[3] = {p->nx, p->ny, p->nz};
-  for (int dir = 0; dir < 3; dir++) {
-    const int order = dir == 0 ? 0 : 1;
-    const int vec = dir == 0 && (p->nx * sizeof(T)) % 16 == 0;
+  auto stage = [&](int g) { (void)stage_t<T>(p, u, du, g, st); };
+  auto x_stage = [&]() {
+    if (use_fused(p, u, du)) {
+      stage(4);
+    } else {
+      stage(0);
+      stage(1);
+    }
+  };
+  // generic x sweep (the unfused path): timed on its own
+  {
+    const int vec = (p->nx * sizeof(T)) % 16 == 0;
     std::vector<PartCfg> cands;
-    for (PartCfg c : candidates(order, n[dir], vec, sizeof(T), true, p->desc[dir].nsys)) {
-      if (dir == 0 && c.vec != 2) continue;  // x keeps the staged layout
-      c.idx32 = p->cfg[dir].idx32;
+    for (PartCfg c : candidates(0, n[0], vec, sizeof(T), true, p->desc[0].nsys)) {
+      if (c.vec != 2) continue;  // x keeps the staged layout
+      c.idx32 = p->cfg[0].idx32;
       cands.push_back(c);
     }
-    // y and z: also the factor-reuse kernel at every chunk length that fits
-    std::vector<int> ccR, ccL;
-    std::vector<void *> ccF;
-    if (dir > 0) {
-      for (int R : {8, 16, 32}) {
-        int cl = 0;
-        void *F = nullptr;
-        if (!cc_rows_ok(n[dir], R, &cl)) continue;
-        if (cc_build<T>(p, dir, R, &F, &cl) != TRID_STATUS_SUCCESS) continue;
-        ccR.push_back(R);
-        ccL.push_back(cl);
-        ccF.push_back(F);
-      }
-    }
-    const int G = (int)cands.size();
-    // candidate G + 2j + x: factor-reuse table j, XCD-contiguous order x
-    auto run = [&](int i) {
-      if (i >= G) {
-        const int j = (i - G) >> 1, x = (i - G) & 1;
-        return dir == 2 ? launch_cc_t<T, 1>(ccR[j], ccL[j], x, ccF[j], p->desc[dir],
-                                            p->cfg[dir].idx32, du, u, st)
-                        : launch_cc_t<T, 0>(ccR[j], ccL[j], x, ccF[j], p->desc[dir],
-                                            p->cfg[dir].idx32, du, du, st);
-      }
-      return dir == 2 ? launch_part<T, 1>(cands[i], cf, du, u, p->desc[dir], st)
-                      : launch_part<T, 0>(cands[i], cf, du, du, p->desc[dir], st);
-    };
-    const int i = pick_fastest(G + 2 * (int)ccR.size(), [](int) { return true; }, run, st);
-    if (i >= 0 && i < G) p->cfg[dir] = cands[i];
-    (void)hipStreamSynchronize(st);
-    if (i >= G) {
-      const int j = (i - G) >> 1;
-      if (p->cc_F[dir]) (void)hipFree(p->cc_F[dir]);
-      p->cc_R[dir] = ccR[j];
-      p->cc_cps_log2[dir] = ccL[j];
-      p->cc_xcd[dir] = (i - G) & 1;
-      p->cc_F[dir] = ccF[j];
-      ccF[j] = nullptr;
-    } else if (i >= 0 && p->cc_F[dir]) {
-      (void)hipFree(p->cc_F[dir]);
-      p->cc_F[dir] = nullptr;
-      p->cc_R[dir] = 0;
-    }
-    for (void *F : ccF)
-      if (F) (void)hipFree(F);
+    auto run = [&](int i) { return launch_part<T, 0>(cands[i], cf, du, du, p->desc[0], st); };
+    const int i = pick_fastest((int)cands.size(), [](int) { return true; }, run, st);
+    if (i >= 0) p->cfg[0] = cands[i];
   }
+  // y and z: generic kernels and the factor-reuse kernel at every chunk
+  // length that fits, in both work-group orders
+  for (int dir = 1; dir < 3; dir++) {
+    struct Choice {
+      int cc, R, cl, xcd;
+      PartCfg cfg;
+      void *F;
+    };
+    std::vector<Choice> ch;
+    for (PartCfg c : candidates(1, n[dir], 0, sizeof(T), true, p->desc[dir].nsys)) {
+      c.idx32 = p->cfg[dir].idx32;
+      ch.push_back({0, 0, 0, 0, c, nullptr});
+    }
+    if (p->cc_F[dir]) (void)hipFree(p->cc_F[dir]);
+    p->cc_F[dir] = nullptr;
+    p->cc_R[dir] = 0;
+    std::vector<void *> tables;
+    for (int R : {8, 16, 32}) {
+      int cl = 0;
+      void *F = nullptr;
+      if (!cc_rows_ok(n[dir], R, &cl)) continue;
+      if (cc_build<T>(p, dir, R, &F, &cl) != TRID_STATUS_SUCCESS) continue;
+      tables.push_back(F);
+      ch.push_back({1, R, cl, 0, p->cfg[dir], F});
+      ch.push_back({1, R, cl, 1, p->cfg[dir], F});
+    }
+    if (ch.empty()) continue;
+    const PartCfg keep = p->cfg[dir];
+    auto apply = [&](const Choice &c) {
+      if (c.cc) {
+        p->cc_R[dir] = c.R;
+        p->cc_cps_log2[dir] = c.cl;
+        p->cc_xcd[dir] = c.xcd;
+        p->cc_F[dir] = c.F;
+        p->cfg[dir] = keep;
+      } else {
+        p->cc_R[dir] = 0;
+        p->cc_F[dir] = nullptr;
+        p->cfg[dir] = c.cfg;
+      }
+    };
+    auto prefix = [&]() {
+      x_stage();
+      if (dir == 2) stage(2);
+    };
+    auto cand = [&](int i) {
+      apply(ch[i]);
+      return stage_t<T>(p, u, du, dir == 1 ? 2 : 3, st) == TRID_STATUS_SUCCESS
+                 ? hipSuccess
+                 : hipErrorLaunchFailure;
+    };
+    auto suffix = [&]() {
+      if (dir == 1) stage(3);
+    };
+    // the stages around the candidate use the current choices of the
+    // other directions; start from the first candidate of this one
+    apply(ch[0]);
+    const int i = pick_in_step((int)ch.size(), prefix, cand, suffix, st);
+    (void)hipStreamSynchronize(st);
+    apply(ch[i >= 0 ? i : 0]);
+    for (void *F : tables)
+      if (F != p->cc_F[dir]) (void)hipFree(F);
+  }
+  // chunk length of the fused RHS + x kernel
   if (p->fused_R) {
     std::vector<int> rows;
     for (int R : {4, 8, 16})
       if (fused_rows_ok(p, R)) rows.push_back(R);
-    auto run = [&](int i) {
+    auto cand = [&](int i) {
       set_fused(p, rows[i]);
       return launch_fused<T>(p, u, du, st);
     };
-    const int i = pick_fastest((int)rows.size(), [](int) { return true; }, run, st);
-    set_fused(p, i >= 0 ? rows[i] : p->fused_R);
+    auto suffix = [&]() {
+      stage(2);
+      stage(3);
+    };
+    const int i = pick_in_step((int)rows.size(), []() {}, cand, suffix, st);
+    set_fused(p, i >= 0 ? rows[i] : rows[0]);
   }
   (void)hipMemcpyAsync(u, save, bytes, hipMemcpyDeviceToDevice, st);
   (void)hipMemcpyAsync(du, save + npts, bytes, hipMemcpyDeviceToDevice, st);
@@ -538,6 +624,10 @@ EXTERN_C tridStatus_t tridAdiPlanCreate(tridAdiPlan_t *plan, int nx, int ny,
     }
     c.idx32 = idx32;
     p->cfg[dir] = c;
+    if (c.kind != 1 || (dir == 0 && p->cfg_x_unaligned.kind != 1)) {
+      delete p;
+      return fail(TRID_STATUS_INVALID_VALUE, "ADI grid dimensions must be at most 8192");
+    }
   }
   // fused RHS + x sweep: whole-chunk lines in one wave, waves within a plane;
   // by default with the x sweep's chunk length, else the first that fits

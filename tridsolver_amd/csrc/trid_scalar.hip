@@ -143,16 +143,15 @@ PartCfg autotune(int order, int vec, int idx32, bool dense, const T *a,
   best.idx32 = idx32;
   if (cands.size() < 2) return best;
   // the candidates run on the caller's own arrays (memory-channel effects of
-  // large power-of-two strides depend on the actual addresses); the written
-  // array is saved first, restored before every candidate and at the end
+  // large power-of-two strides depend on the actual addresses), repeatedly
+  // and in place as a caller runs them; the written array is saved first and
+  // restored at the end
   T *saved = static_cast<T *>(workspace((size_t)d_extent * sizeof(T)));
   if (!saved) return best;
   T *dst = INC ? u : d;
   const size_t bytes = (size_t)d_extent * sizeof(T);
   if (hipMemcpyAsync(saved, dst, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess) return best;
-  auto prep = [&](int) {
-    return hipMemcpyAsync(dst, saved, bytes, hipMemcpyDeviceToDevice, st) == hipSuccess;
-  };
+  auto prep = [&](int) { return true; };
   auto run = [&](int i) {
     return launch_part<T, INC>(cands[i], CoefArrays<T>{a, b, c}, d, u, D, st);
   };
