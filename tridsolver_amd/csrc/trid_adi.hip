@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -132,21 +133,37 @@ __global__ void __launch_bounds__(256)
     span_issue<T, R>(u + wo + nx, lane, syp);
     span_issue<T, R>(u + wo - nxy, lane, szm);
     span_issue<T, R>(u + wo + nxy, lane, szp);
-    T uc[R], ul, ur;
-    span_stage<T, R>(sc, stage, lane, uc, &ul, &ur);
-    T ym[R], yp[R], zm[R], zp[R];
-    span_stage<T, R>(sym, stage, lane, ym);
-    span_stage<T, R>(syp, stage, lane, yp);
-    span_stage<T, R>(szm, stage, lane, zm);
-    span_stage<T, R>(szp, stage, lane, zp);
+    // the stencil is evaluated in the loaded layout (lane holds elements
+    // (v*64 + lane)*E .. +E-1 of the wave's span): the neighbour spans line
+    // up element for element, the x-neighbours at piece ends come from the
+    // adjacent lanes; only the right-hand side is transposed into chunks
+    constexpr int E = Vec16<T>::n, NP = R / E;
+    const int nx_log2 = __builtin_ctz(nx);  // nx = cps * R, a power of two
+    const int jw = (int)(wline0 % ny);      // the wave's first line in its plane
+    Span<T, R> sd;
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-      const int row = row0 + r;
-      const bool in = line_in && row >= 1 && row + 2 <= nx;
-      const T l = r == 0 ? ul : uc[r - 1];
-      const T rr = r == R - 1 ? ur : uc[r + 1];
-      d[r] = in ? rhs_point(l, rr, ym[r], yp[r], zm[r], zp[r], uc[r], lam) : T(0);
+    for (int v = 0; v < NP; v++) {
+      const auto cv = sc.v[v];
+      T left = lane_read(cv[E - 1], lane > 0 ? lane - 1 : 0);
+      T right = lane_read(cv[0], lane < kWave - 1 ? lane + 1 : kWave - 1);
+      const T prev_last = v > 0 ? lane_bcast(sc.v[v > 0 ? v - 1 : 0][E - 1], kWave - 1) : T(0);
+      const T next_first = v + 1 < NP ? lane_bcast(sc.v[v + 1 < NP ? v + 1 : 0][0], 0) : T(0);
+      if (lane == 0) left = prev_last;
+      if (lane == kWave - 1) right = next_first;
+#pragma unroll
+      for (int j = 0; j < E; j++) {
+        const int e = (v * kWave + lane) * E + j;
+        const int row = e & (nx - 1);
+        const int jl_e = jw + (e >> nx_log2);
+        const bool in = jl_e >= 1 && jl_e <= ny - 2 && row >= 1 && row + 2 <= nx;
+        const T l = j == 0 ? left : cv[j > 0 ? j - 1 : 0];
+        const T rr = j == E - 1 ? right : cv[j + 1 < E ? j + 1 : 0];
+        sd.v[v][j] = in ? rhs_point(l, rr, sym.v[v][j], syp.v[v][j], szm.v[v][j], szp.v[v][j],
+                                    cv[j], lam)
+                        : T(0);
+      }
     }
+    span_stage<T, R>(sd, stage, lane, d);
   }
   Row<T> f, l;
   chunk_reduce<T, R>(a, b, c, d, f, l);
@@ -389,7 +406,7 @@ void set_fused(tridAdiPlan_st *p, int R) {
 // time of its best round (-1 if none ran).
 template <typename Prefix, typename Cand, typename Suffix>
 int pick_in_step(int ncand, Prefix prefix, Cand cand, Suffix suffix, hipStream_t st,
-                 int reps = 4, int passes = 2) {
+                 std::vector<float> *times = nullptr, int reps = 4, int passes = 2) {
   std::vector<hipEvent_t> ev(2 * reps, nullptr);
   for (auto &e : ev)
     if (hipEventCreate(&e) != hipSuccess) {
@@ -425,10 +442,17 @@ int pick_in_step(int ncand, Prefix prefix, Cand cand, Suffix suffix, hipStream_t
     }
   for (auto &e : ev) (void)hipEventDestroy(e);
   (void)hipGetLastError();
+  if (times) *times = best;
   int arg = -1;
   for (int i = 0; i < ncand; i++)
     if (best[i] < 1e29f && (arg < 0 || best[i] < best[arg])) arg = i;
   return arg;
+}
+
+// TRIDHIP_TUNE_LOG=1: the plan tuner prints every candidate's time.
+bool tune_log() {
+  const char *e = std::getenv("TRIDHIP_TUNE_LOG");
+  return e && e[0] == '1';
 }
 
 // Tunes the plan on the caller's own arrays (the speed of large power-of-two
@@ -527,7 +551,14 @@ void tune_plan_t(tridAdiPlan_st *p, T *u, T *du) {
     // the stages around the candidate use the current choices of the
     // other directions; start from the first candidate of this one
     apply(ch[0]);
-    const int i = pick_in_step((int)ch.size(), prefix, cand, suffix, st);
+    std::vector<float> tm;
+    const int i = pick_in_step((int)ch.size(), prefix, cand, suffix, st, &tm);
+    if (tune_log())
+      for (size_t c = 0; c < ch.size(); c++)
+        fprintf(stderr, "[tridhip tune] %s sweep %s%s: %.4f ms\n", dir == 1 ? "y" : "z",
+                ch[c].cc ? ("cc<R" + std::to_string(ch[c].R) + ">").c_str()
+                         : cfg_name(ch[c].cfg, sizeof(T)).c_str(),
+                ch[c].cc && ch[c].xcd ? ",xcd" : "", tm[c]);
     (void)hipStreamSynchronize(st);
     apply(ch[i >= 0 ? i : 0]);
     for (void *F : tables)
@@ -546,7 +577,11 @@ void tune_plan_t(tridAdiPlan_st *p, T *u, T *du) {
       stage(2);
       stage(3);
     };
-    const int i = pick_in_step((int)rows.size(), []() {}, cand, suffix, st);
+    std::vector<float> tm;
+    const int i = pick_in_step((int)rows.size(), []() {}, cand, suffix, st, &tm);
+    if (tune_log())
+      for (size_t c = 0; c < rows.size(); c++)
+        fprintf(stderr, "[tridhip tune] rhs+x fused R%d: %.4f ms\n", rows[c], tm[c]);
     set_fused(p, i >= 0 ? rows[i] : rows[0]);
   }
   (void)hipMemcpyAsync(u, save, bytes, hipMemcpyDeviceToDevice, st);
@@ -563,8 +598,9 @@ std::string plan_text(const tridAdiPlan_st *p) {
     return "cc<R" + std::to_string(p->cc_R[dir]) + ",cps" +
            std::to_string(1 << p->cc_cps_log2[dir]) + (p->cc_xcd[dir] ? ",xcd>" : ">");
   };
-  return std::string(p->fused_R ? "rhs+x(fused,R" + std::to_string(p->fused_R) + ")<"
-                                 : "rhs<") +
+  std::string fx = "rhs<";
+  if (p->fused_R) fx = "rhs+x(fused,R" + std::to_string(p->fused_R) + ")<";
+  return fx +
          (p->dbl ? "double" : "float") + "> x:" + cfg_name(p->cfg[0], ts) +
          " y:" + sweep_name(1) + " z:" + sweep_name(2);
 }

@@ -96,6 +96,16 @@ __device__ __forceinline__ double lane_read(double v, int src) {
   return __hiloint2double(hi, lo);
 }
 
+// Value of lane `src` (wave-uniform) broadcast to the whole wave.
+__device__ __forceinline__ float lane_bcast(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ double lane_bcast(double v, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+  return __hiloint2double(hi, lo);
+}
+
 // One normalised tridiagonal row (diagonal 1): a x_{-} + x + c x_{+} = d.
 template <typename T>
 struct Row {
