@@ -134,43 +134,77 @@ __global__ void __launch_bounds__(256)
       span_stage<double, R>(sk, stage, lane, kc, &kl, &kr);
       if (MODE == 1) span_stage<double, R>(sd, stage, lane, d);
     } else {
-      // d = lam * (x part + y part + z part) of the stencil; the global loads
-      // of the next neighbour pair are issued before the current pair is
-      // transposed (two pairs in flight)
+      // d = lam * (x part + y part + z part) of the stencil, evaluated in the
+      // loaded layout (lane holds elements (v*64 + lane)*2 .. +1 of the
+      // wave's span): neighbour spans line up element for element, the
+      // x-neighbours at piece ends come from the adjacent lanes; only kappa
+      // (for the coefficients) and the right-hand side are transposed into
+      // chunks.
+      constexpr int NP = R / 2;
       const long long nb[4] = {-(long long)nx, (long long)nx, -nxy, nxy};
-      Span<double, R> sk, su, pu[2], pk[2];
+      Span<double, R> sk, su, pu, pk, sd;
       span_issue<double, R>(K + wo, lane, sk);
       span_issue<double, R>(in + wo, lane, su);
-      span_issue<double, R>(in + wo + nb[0], lane, pu[0]);
-      span_issue<double, R>(K + wo + nb[0], lane, pk[0]);
-      double uc[R], ul, ur;
+      span_issue<double, R>(in + wo + nb[0], lane, pu);
+      span_issue<double, R>(K + wo + nb[0], lane, pk);
       span_stage<double, R>(sk, stage, lane, kc, &kl, &kr);
-      span_stage<double, R>(su, stage, lane, uc, &ul, &ur);
+      const int nx_log2 = __builtin_ctz(nx);  // nx = cps * R, a power of two
+      const int jw = (int)(wline0 % ny);      // the wave's first line in its plane
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        const double kpv = r == 0 ? kl : kc[r - 1], knx = r == R - 1 ? kr : kc[r + 1];
-        const double upv = r == 0 ? ul : uc[r - 1], unx = r == R - 1 ? ur : uc[r + 1];
-        const double kp = 0.5 * (kc[r] + knx), km = 0.5 * (kc[r] + kpv);
-        d[r] = kp * (unx - uc[r]) - km * (uc[r] - upv);
+      for (int v = 0; v < NP; v++) {
+        const auto uv = su.v[v], kv = sk.v[v];
+        double ul = lane_read(uv[1], lane > 0 ? lane - 1 : 0);
+        double ur = lane_read(uv[0], lane < kWave - 1 ? lane + 1 : kWave - 1);
+        double kl_ = lane_read(kv[1], lane > 0 ? lane - 1 : 0);
+        double kr_ = lane_read(kv[0], lane < kWave - 1 ? lane + 1 : kWave - 1);
+        const double up = v > 0 ? lane_bcast(su.v[v > 0 ? v - 1 : 0][1], kWave - 1) : 0.0;
+        const double un = v + 1 < NP ? lane_bcast(su.v[v + 1 < NP ? v + 1 : 0][0], 0) : 0.0;
+        const double kp_ = v > 0 ? lane_bcast(sk.v[v > 0 ? v - 1 : 0][1], kWave - 1) : 0.0;
+        const double kn_ = v + 1 < NP ? lane_bcast(sk.v[v + 1 < NP ? v + 1 : 0][0], 0) : 0.0;
+        if (lane == 0) {
+          ul = up;
+          kl_ = kp_;
+        }
+        if (lane == kWave - 1) {
+          ur = un;
+          kr_ = kn_;
+        }
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          const double upv = j == 0 ? ul : uv[0], unx = j == 1 ? ur : uv[1];
+          const double kpv = j == 0 ? kl_ : kv[0], knx = j == 1 ? kr_ : kv[1];
+          const double kp = 0.5 * (kv[j] + knx), km = 0.5 * (kv[j] + kpv);
+          sd.v[v][j] = kp * (unx - uv[j]) - km * (uv[j] - upv);
+        }
       }
 #pragma unroll
       for (int t = 0; t < 4; t++) {
-        if (t + 1 < 4) {
-          span_issue<double, R>(in + wo + nb[t + 1], lane, pu[(t + 1) & 1]);
-          span_issue<double, R>(K + wo + nb[t + 1], lane, pk[(t + 1) & 1]);
+        if (t > 0) {
+          span_issue<double, R>(in + wo + nb[t], lane, pu);
+          span_issue<double, R>(K + wo + nb[t], lane, pk);
         }
-        double un[R], kn[R];
-        span_stage<double, R>(pu[t & 1], stage, lane, un);
-        span_stage<double, R>(pk[t & 1], stage, lane, kn);
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-          const double kf = 0.5 * (kc[r] + kn[r]);
-          // minus side: -kf (u - u-), plus side: +kf (u+ - u)
-          d[r] += (t & 1) ? kf * (un[r] - uc[r]) : -(kf * (uc[r] - un[r]));
-        }
+        for (int v = 0; v < NP; v++)
+#pragma unroll
+          for (int j = 0; j < 2; j++) {
+            const double uc_ = su.v[v][j], kc_ = sk.v[v][j];
+            const double un_ = pu.v[v][j], kn_ = pk.v[v][j];
+            const double kf = 0.5 * (kc_ + kn_);
+            // minus side: -kf (u - u-), plus side: +kf (u+ - u)
+            sd.v[v][j] += (t & 1) ? kf * (un_ - uc_) : -(kf * (uc_ - un_));
+          }
       }
 #pragma unroll
-      for (int r = 0; r < R; r++) d[r] = inner(row0 + r) ? lam * d[r] : 0.0;
+      for (int v = 0; v < NP; v++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          const int e = (v * kWave + lane) * 2 + j;
+          const int row = e & (nx - 1);
+          const int jl_e = jw + (e >> nx_log2);
+          const bool in_e = plane_in && jl_e >= 1 && jl_e <= ny - 2 && row >= 1 && row + 2 <= nx;
+          sd.v[v][j] = in_e ? lam * sd.v[v][j] : 0.0;
+        }
+      span_stage<double, R>(sd, stage, lane, d);
     }
   }
   double a[R], b[R], c[R];
@@ -361,7 +395,7 @@ tridStatus_t vadi_rhs(const double *K, const double *u, double *r, const Grid &g
 // Chunk length of the staged x kernel for this grid (0: not applicable).
 int staged_x_rows(const Grid &g, const void *K, const void *in, const void *out) {
   if (!aligned16(K) || !aligned16(in) || !aligned16(out)) return 0;
-  for (int R : {8, 4, 16}) {
+  for (int R : {4, 8, 16}) {
     if (g.nx % R) continue;
     const int cps = g.nx / R;
     if (cps > kWave || (cps & (cps - 1))) continue;
