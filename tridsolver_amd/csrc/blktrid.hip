@@ -20,6 +20,7 @@
 // block row n of A^T is (C_{n-1}^T, B_n^T, A_{n+1}^T).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstdio>
 #include <cstring>
@@ -219,6 +220,205 @@ constexpr long long blk_blocks(long long P) {
   return (P + 4 * (kWave / L) - 1) / (4 * (kWave / L));
 }
 
+// ---- one lane per block column, tiles through LDS ---------------------------
+// The default layout (L = M <= 8).  Every M x M block is moved between
+// global memory and the group's LDS tile cooperatively -- lane j of a group
+// takes elements j, j+M, j+2M, ... of the contiguous block, so each load or
+// store instruction of a wave covers one contiguous stretch of 64 elements --
+// and every lane then reads what it needs from the tile: column j of B_n and
+// C_n, every entry of A_n (group-wide broadcast reads), row i of C'_n in the
+// backward sweep.  The forward recurrence, Gauss-Jordan elimination and
+// storage format are those of blk_problem; the backward sweep is
+// row-distributed (lane i forms u_n[i] = d'_n[i] - C'_n[i,:] u_{n+1}, and u_n
+// is shared through the tile).  The groups of a wave exchange data only
+// within their own wave, so wave barriers order the LDS accesses.
+template <typename T, int M>
+constexpr int blk2_tile() {
+  return 3 * M * M + 2 * M;  // A, B, C tiles, d and u vectors
+}
+
+// p: the problem (problem P-1 for the lanes of groups past the end, which
+// solve it again and store nothing); store: whether the lane writes results;
+// lane_ok: the lane belongs to a whole group (idle lanes do not write the
+// tile).
+template <typename T, int M, int TRANS>
+__device__ __forceinline__ void blk2_problem(const BlkArgs<T> &g, long long p, bool store,
+                                             bool lane_ok, int j, int gbase, T *tile) {
+  const long long P = g.P;
+  const int N = g.N;
+  auto blk = [&](int n) { return ((long long)n * P + p) * (M * M); };
+  auto vec = [&](int n) { return ((long long)n * P + p) * M; };
+  T *sA = tile, *sB = tile + M * M, *sC = tile + 2 * M * M, *sd = tile + 3 * M * M;
+  T *su = sd + M;
+  auto wb = []() { __builtin_amdgcn_wave_barrier(); };
+
+  T Cp[M], dp[M];
+#pragma unroll
+  for (int i = 0; i < M; i++) {
+    Cp[i] = T(0);
+    dp[i] = T(0);
+  }
+  for (int n = 0; n < N; n++) {
+    // the row's tiles: sub-diagonal (A_n; C_{n-1} transposed), diagonal
+    // (B_n; transposed), super-diagonal (C_n; A_{n+1} transposed)
+    const bool has_sub = n > 0, has_sup = n < N - 1;
+    T ra[M], rb[M], rc[M], rd;
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+      const int e = j + M * r;
+      ra[r] = has_sub ? (TRANS ? g.C[blk(n - 1) + e] : g.A[blk(n) + e]) : T(0);
+      rb[r] = g.B[blk(n) + e];
+      rc[r] = has_sup ? (TRANS ? g.A[blk(n + 1) + e] : g.C[blk(n) + e]) : T(0);
+    }
+    rd = g.d[vec(n) + j];
+    if (lane_ok) {
+#pragma unroll
+      for (int r = 0; r < M; r++) {
+        const int e = j + M * r;
+        sA[e] = ra[r];
+        sB[e] = rb[r];
+        sC[e] = rc[r];
+      }
+      sd[j] = rd;
+    }
+    wb();
+    // element (i, c) of the row's sub/diag/sup block
+    auto at_t = [&](const T *t, int i, int c) { return TRANS ? t[c * M + i] : t[i * M + c]; };
+    T X[M], Cn[M], v[M];
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+      X[i] = at_t(sB, i, j);
+      Cn[i] = at_t(sC, i, j);
+      v[i] = sd[i];
+    }
+    if (has_sub) {
+#pragma unroll
+      for (int k = 0; k < M; k++) {
+#pragma unroll
+        for (int i = 0; i < M; i++) {
+          const T aik = at_t(sA, i, k);
+          X[i] -= aik * Cp[k];
+          v[i] -= aik * dp[k];
+        }
+        // keep a few columns of A_n in flight: the scheduler would otherwise
+        // hoist all M*M tile reads and spill for large blocks
+        if constexpr (M >= 5) {
+          if ((k & 1) == 1) __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+    }
+    wb();
+    // phase boundary: no tile load or read moves across the elimination
+    __builtin_amdgcn_sched_barrier(0);
+    // Gauss-Jordan on [X | Cn | v] without pivoting (pivot columns from the
+    // owning lane)
+#pragma unroll
+    for (int q = 0; q < M; q++) {
+      T col[M];
+#pragma unroll
+      for (int i = 0; i < M; i++) col[i] = lane_read(X[i], min(gbase + q, kWave - 1));
+      const T inv = rcp(col[q]);
+      X[q] *= inv;
+      Cn[q] *= inv;
+      v[q] *= inv;
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+        if (i == q) continue;
+        const T f = col[i];
+        X[i] -= f * X[q];
+        Cn[i] -= f * Cn[q];
+        v[i] -= f * v[q];
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    // C'_n (column j in this lane) and d'_n to the caller's scratch
+    if (has_sup) {
+      if (lane_ok) {
+#pragma unroll
+        for (int i = 0; i < M; i++) sC[i * M + j] = Cn[i];
+      }
+      wb();
+      if (store) {
+#pragma unroll
+        for (int r = 0; r < M; r++) g.Cs[blk(n) + j + M * r] = sC[j + M * r];
+      }
+      wb();
+    }
+    if (store) g.ds[vec(n) + j] = v[j];
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+      Cp[i] = Cn[i];
+      dp[i] = v[i];
+    }
+  }
+  // backward: u_{N-1} = d'_{N-1} (dp, in every lane), u_n = d'_n - C'_n u_{n+1}
+  T un[M];
+#pragma unroll
+  for (int i = 0; i < M; i++) un[i] = dp[i];
+  if (store) g.u[vec(N - 1) + j] = un[j];
+  for (int n = N - 2; n >= 0; n--) {
+    // C'_n and d'_n as this lane's group stored them (stores by the other
+    // lanes of a wave are visible to its later loads)
+    T rc[M];
+#pragma unroll
+    for (int r = 0; r < M; r++) rc[r] = g.Cs[blk(n) + j + M * r];
+    const T dj = g.ds[vec(n) + j];
+    if (lane_ok) {
+#pragma unroll
+      for (int r = 0; r < M; r++) sC[j + M * r] = rc[r];
+    }
+    wb();
+    T acc = dj;
+#pragma unroll
+    for (int k = 0; k < M; k++) acc -= sC[j * M + k] * un[k];
+    if (lane_ok) su[j] = acc;
+    wb();
+#pragma unroll
+    for (int k = 0; k < M; k++) un[k] = su[k];
+    wb();
+    if (store) g.u[vec(n) + j] = acc;
+  }
+}
+
+// 256 threads = 4 waves of 64/M groups of M lanes (lanes past the last whole
+// group are idle)
+template <typename T, int M, int TRANS>
+__device__ __forceinline__ void blk2_block(const BlkArgs<T> &g, long long block, T *smem) {
+  constexpr int G = kWave / M;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x >> 6;
+  const int grp = lane / M;
+  const int j = lane - grp * M;
+  const long long p = (block * 4 + wave) * G + grp;
+  const bool lane_ok = grp < G;
+  const bool store = lane_ok && p < g.P;
+  T *tile = smem + (wave * G + (lane_ok ? grp : G - 1)) * blk2_tile<T, M>();
+  blk2_problem<T, M, TRANS>(g, std::min<long long>(p, g.P - 1), store, lane_ok, j, grp * M, tile);
+}
+
+// Minimum waves per SIMD for the register allocator (the live set is about
+// 5M values per lane; without a bound the scheduler hoists tile loads and
+// doubles it).
+constexpr int blk2_min_waves(int M) {
+  return M <= 2 ? 8 : (M == 3 ? 5 : (M == 4 ? 4 : (M == 5 ? 3 : 2)));
+}
+
+template <typename T, int M, int TRANS>
+__global__ void __launch_bounds__(256, blk2_min_waves(M)) k_blk2(BlkArgs<T> g) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  blk2_block<T, M, TRANS>(g, blockIdx.x, reinterpret_cast<T *>(smem_raw));
+}
+
+// Block sizes for which the tiled kernel is the default (measured faster on
+// MI355X; for M = 1, 2 the direct kernel's occupancy wins, for M = 7, 8 the
+// tiled kernel's register demand does not fit).
+__host__ __device__ constexpr bool blk2_sizes(int M) { return M >= 3 && M <= 6; }
+
+template <typename T, int M>
+constexpr size_t blk2_lds() {
+  return (size_t)4 * (kWave / M) * blk2_tile<T, M>() * sizeof(T);
+}
+
 // ---- mixed block sizes in one launch ---------------------------------------
 constexpr int kMaxGroups = 16;
 struct MixedArgs {
@@ -233,7 +433,9 @@ __host__ __device__ constexpr int default_lanes(int M) {
   return M == 1 ? 1 : (M == 2 ? 2 : (M == 3 ? 3 : (M == 4 ? 4 : (M == 8 ? 8 : M))));
 }
 
-__global__ void __launch_bounds__(256, 2) k_blk_mixed(MixedArgs a) {
+__global__ void __launch_bounds__(256) k_blk_mixed(MixedArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double *smem = reinterpret_cast<double *>(smem_raw);
   const long long b = blockIdx.x;
   int gi = 0;
   while (gi + 1 < a.ngroups && b >= a.first_block[gi + 1]) gi++;
@@ -241,7 +443,10 @@ __global__ void __launch_bounds__(256, 2) k_blk_mixed(MixedArgs a) {
   switch (a.M[gi]) {
 #define MIXED_CASE(MM)                                                            \
   case MM:                                                                        \
-    blk_block<double, MM, default_lanes(MM), 0>(a.g[gi], lb);                     \
+    if constexpr (blk2_sizes(MM))                                                 \
+      blk2_block<double, MM, 0>(a.g[gi], lb, smem);                               \
+    else                                                                          \
+      blk_block<double, MM, default_lanes(MM), 0>(a.g[gi], lb);                   \
     break;
     MIXED_CASE(1)
     MIXED_CASE(2)
@@ -291,6 +496,17 @@ template <typename T, int M, int TRANS>
 hipError_t launch_m(const BlkArgs<T> &g, int lanes, hipStream_t st) {
   constexpr int Ld = default_lanes(M);
   const int L = lanes ? lanes : Ld;
+  if constexpr (blk2_sizes(M)) {
+    // the default layout for these block sizes: one lane per block column,
+    // tiles through LDS
+    if (lanes == 0) {
+      const long long G = 4 * (kWave / M);
+      const size_t lds = blk2_lds<T, M>();
+      hipLaunchKernelGGL((k_blk2<T, M, TRANS>), dim3((unsigned)((g.P + G - 1) / G)), dim3(256),
+                         lds, st, g);
+      return hipGetLastError();
+    }
+  }
 #define LCASE(LL)                                                                \
   if constexpr (M % LL == 0)                                                     \
     if (L == LL) {                                                               \
@@ -420,8 +636,15 @@ EXTERN_C tridStatus_t tridDbtsvMixedBatch(const tridDbtsvGroup_t *groups,
   a.ngroups = k;
   a.first_block[k] = nb;
   if (nb == 0) return TRID_STATUS_SUCCESS;
+  // LDS: the largest tile set of the batch's block sizes
+  size_t lds = 0;
+  for (int i = 0; i < k; i++) {
+    if (!blk2_sizes(a.M[i])) continue;
+    const size_t b = (size_t)4 * (kWave / a.M[i]) * (3 * a.M[i] * a.M[i] + 2 * a.M[i]) * 8;
+    lds = std::max(lds, b);
+  }
   hipStream_t st = current_stream();
-  hipLaunchKernelGGL(k_blk_mixed, dim3((unsigned)nb), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_blk_mixed, dim3((unsigned)nb), dim3(256), lds, st, a);
   tridStatus_t s = check_hip(hipGetLastError(), "mixed block kernel launch");
   if (s) return s;
   if (sync) return check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
