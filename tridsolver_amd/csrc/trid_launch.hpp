@@ -61,7 +61,9 @@ hipError_t launch_part_t(const PartCfg &cfg, const Coef &coef, T *d, T *u,
   const int threads = cps * spb;
   const long long blocks = (D.nsys + spb - 1) / spb;
   size_t lds = 0;
-  if (ORDER == 1 || cps > kWave) lds = 2 * 3 * (size_t)cps * spb * sizeof(T);
+  // reduced-system exchange: reduced_lds (6 * cps * spb values) or, for
+  // ORDER 1 with cps <= 64, reduced_lds_w (6 * cps * (spb + 1))
+  if (ORDER == 1 || cps > kWave) lds = 2 * 3 * (size_t)cps * (spb + 1) * sizeof(T);
   if (VEC == 2) lds = (size_t)threads * R * sizeof(T);  // wave staging buffers
   if (cfg.idx32)
     hipLaunchKernelGGL((k_part<T, INC, ORDER, R, VEC, uint32_t, Coef>),
@@ -199,8 +201,8 @@ inline std::vector<PartCfg> candidates(int order, int N, int vec, size_t tsize,
 // timed run(i) launches on st.  Returns the index with the lowest time of
 // its best round (-1 if no candidate could be launched).
 template <typename Prep, typename Run>
-int pick_fastest(int ncand, Prep prep, Run run, hipStream_t st, int reps = 5,
-                 int passes = 2) {
+int pick_fastest(int ncand, Prep prep, Run run, hipStream_t st,
+                 std::vector<float> *times = nullptr, int reps = 5, int passes = 2) {
   std::vector<float> best(ncand, 1e30f);
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess) return -1;
@@ -227,6 +229,7 @@ int pick_fastest(int ncand, Prep prep, Run run, hipStream_t st, int reps = 5,
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipGetLastError();
+  if (times) *times = best;
   int arg = -1;
   for (int i = 0; i < ncand; i++)
     if (best[i] < 1e29f && (arg < 0 || best[i] < best[arg])) arg = i;

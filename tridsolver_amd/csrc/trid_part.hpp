@@ -194,6 +194,45 @@ __device__ __forceinline__ void reduced_lds(const Row<T> &f, const Row<T> &l,
   }
 }
 
+// Same, for chunks spread over several waves with cps <= 64: the chunks'
+// two reduced rows go to LDS, every wave then takes whole systems (64 / cps
+// systems, chunks in consecutive lanes) and solves them with the in-wave
+// PCR of reduced_wave (no barrier per level), and the chunks' unknowns go
+// back through LDS -- three barriers in all instead of one per PCR level.
+// Entry (chunk k, system slot s) lives at k * (spb + 1) + s of six arrays of
+// cps * (spb + 1) values (the padding keeps the wave-local reads, which walk
+// k with s fixed, free of bank conflicts).  All threads call it; blockDim =
+// cps * spb.
+template <typename T>
+__device__ __forceinline__ void reduced_lds_w(const Row<T> &f, const Row<T> &l, int k, int cps,
+                                              int slot, int spb, T *sm, T &xf, T &xl) {
+  const int sp = spb + 1;
+  const int n = cps * sp;
+  const int me = k * sp + slot;
+  sm[me] = f.a;
+  sm[n + me] = f.c;
+  sm[2 * n + me] = f.d;
+  sm[3 * n + me] = l.a;
+  sm[4 * n + me] = l.c;
+  sm[5 * n + me] = l.d;
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1);
+  const int cps_log2 = __builtin_ctz(cps);
+  const int k2 = lane & (cps - 1);
+  const int s2 = (int)(threadIdx.x >> 6) * (kWave >> cps_log2) + (lane >> cps_log2);
+  const int e = k2 * sp + s2;
+  const Row<T> f2{sm[e], sm[n + e], sm[2 * n + e]};
+  const Row<T> l2{sm[3 * n + e], sm[4 * n + e], sm[5 * n + e]};
+  T xf2, xl2;
+  reduced_wave<T>(f2, l2, k2, cps, lane, xf2, xl2);
+  __syncthreads();  // every entry has been read
+  sm[e] = xf2;
+  sm[n + e] = xl2;
+  __syncthreads();
+  xf = sm[me];
+  xl = sm[n + me];
+}
+
 // Streaming (non-temporal) global accesses: every element of a, b, c, d
 // and u is touched exactly once per solve, so the loads and stores carry the
 // non-temporal hint.  On MI355X this lifts a read-4/write-1 stream from
@@ -660,6 +699,8 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
   T xf, xl;
   if (ORDER == 0 && cps <= kWave) {
     reduced_wave<T>(f, l, k, cps, tid & (kWave - 1), xf, xl);
+  } else if (cps <= kWave) {
+    reduced_lds_w<T>(f, l, k, cps, slot, spb, reinterpret_cast<T *>(smem_raw), xf, xl);
   } else {
     reduced_lds<T>(f, l, k, cps, slot, spb, reinterpret_cast<T *>(smem_raw),
                    xf, xl);

@@ -11,6 +11,8 @@
 // and a one-thread-per-system fallback for shapes outside its limits.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -103,7 +105,10 @@ hipError_t launch_fallback(const T *a, const T *b, const T *c, T *d, T *u,
 
 
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int, int, long long, int>, PartCfg> g_tuned;
+// tuned configuration per (element size, Inc, order, N, systems, layout,
+// solve stride, first other stride): strides decide the memory pattern (sd1
+// and sd2 of a cube share everything else)
+std::map<std::tuple<int, int, int, int, long long, int, long long, long long>, PartCfg> g_tuned;
 
 // Extent (in elements) of an array with the given dims/pads.
 long long extent(int ndim, const int *dims, const int *pads) {
@@ -155,8 +160,14 @@ PartCfg autotune(int order, int vec, int idx32, bool dense, const T *a,
   auto run = [&](int i) {
     return launch_part<T, INC>(cands[i], CoefArrays<T>{a, b, c}, d, u, D, st);
   };
-  const int i = pick_fastest((int)cands.size(), prep, run, st);
+  std::vector<float> tm;
+  const int i = pick_fastest((int)cands.size(), prep, run, st, &tm);
   if (i >= 0) best = cands[i];
+  const char *log = std::getenv("TRIDHIP_TUNE_LOG");
+  if (log && log[0] == '1')
+    for (size_t c = 0; c < cands.size(); c++)
+      fprintf(stderr, "[tridhip tune] N=%d nsys=%lld order %d %s: %.4f ms\n", D.N, D.nsys,
+              order, cfg_name(cands[c], sizeof(T)).c_str(), tm[c]);
   (void)hipMemcpyAsync(dst, saved, bytes, hipMemcpyDeviceToDevice, st);
   return best;
 }
@@ -250,7 +261,8 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
     cfg.xcd = fc.kind == 3;
   } else if (autotune_mode() == 1) {
     auto key = std::make_tuple((int)sizeof(T), INC, order, D.N, D.nsys,
-                               vec * 2 + (int)dense);
+                               vec * 2 + (int)dense, D.sstr[3],
+                               D.nother > 0 ? D.ostr[3][0] : 0LL);
     bool have = false;
     {
       std::lock_guard<std::mutex> lk(g_tune_mu);
