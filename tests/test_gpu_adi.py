@@ -201,6 +201,49 @@ def test_adi_graph_replays_match_eager():
         np.testing.assert_array_equal(ug.cpu().numpy(), ue.cpu().numpy(), err_msg=f"replay {rep}")
 
 
+@pytest.mark.parametrize("shape", [(64, 40, 24), (128, 64, 48), (6, 600, 5)])
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_adi_autotuned_plan_vs_oracle(shape, dt):
+    """With autotuning on, the plan times its kernel choices at its first use on
+    the caller's u and du (saved and restored): the steps must still give the
+    oracle's answer."""
+    import tridsolver_amd as ta
+    nx, ny, nz = shape
+    u0 = rand_init(nx, ny, nz, dt, seed=11)
+    ta.set_autotune(1)
+    try:
+        got, plan = run_gpu(u0, 5, 1)
+    finally:
+        ta.set_autotune(0)
+    exp = u0.ravel().copy()
+    assert oracle.adi(exp, nx, ny, nz, 5, 1.0, 1) == 0
+    err = oracle.rel_err(exp, got.ravel())
+    assert err <= tol(dt), f"{shape}: {err:.2e} [{plan.describe()}]"
+
+
+def test_adi_autotuned_graph_capture_vs_oracle():
+    """Tuning triggered by the capture call runs before the capture starts; the
+    captured steps replay to the oracle's answer."""
+    import torch
+    import tridsolver_amd as ta
+    nx, ny, nz = 64, 48, 40
+    u0 = rand_init(nx, ny, nz, np.float64, seed=12)
+    ta.set_autotune(1)
+    try:
+        plan = AdiPlan(nx, ny, nz, torch.float64, 1.0, 1)
+        u = torch.from_numpy(u0.copy()).to("cuda")
+        du = torch.empty_like(u)
+        plan.capture(u, du, 2)
+    finally:
+        ta.set_autotune(0)
+    plan.replay()
+    plan.replay()
+    exp = u0.ravel().copy()
+    assert oracle.adi(exp, nx, ny, nz, 4, 1.0, 1) == 0
+    err = oracle.rel_err(exp, u.cpu().numpy().ravel())
+    assert err <= 1e-12, f"{err:.2e} [{plan.describe()}]"
+
+
 def test_adi_invalid_arguments():
     import torch
     from tridsolver_amd import TridError

@@ -215,6 +215,7 @@ struct tridAdiPlan_st {
   int cc_R[3] = {0, 0, 0};
   int cc_cps_log2[3] = {0, 0, 0};
   int cc_xcd[3] = {0, 0, 0};
+  int cc_pair[3] = {0, 0, 0};  // two adjacent systems per lane (16-byte rows)
   void *cc_F[3] = {nullptr, nullptr, nullptr};
   bool tune_pending = false;
   hipGraph_t graph = nullptr;
@@ -297,34 +298,60 @@ tridStatus_t cc_build(tridAdiPlan_st *p, int dir, int R, void **F, int *cps_log2
   return s;
 }
 
+// Two adjacent systems per lane need the systems' first other dimension to
+// be unit-stride in d (and u), an even count of it, and 16-byte alignment.
+template <typename T>
+bool cc_pair_ok(const SysDesc &D, const T *d, const T *u, int inc) {
+  if (D.nother < 1 || D.ostr[3][0] != 1 || D.osize[0] % 2) return false;
+  if (inc && D.ostr[4][0] != 1) return false;
+  if (!aligned16(d) || (inc && !aligned16(u))) return false;
+  return D.nsys % 2 == 0;
+}
+
 template <typename T, int INC>
-hipError_t launch_cc_t(int R, int cps_log2, int xcd, const void *F, const SysDesc &D, int idx32,
-                       T *d, T *u, hipStream_t st) {
+hipError_t launch_cc_t(int R, int cps_log2, int xcd, int pair, const void *F, const SysDesc &D,
+                       int idx32, T *d, T *u, hipStream_t st) {
   const int cps = 1 << cps_log2;
-  const unsigned blocks = (unsigned)((D.nsys + kWave - 1) / kWave);
-  const size_t lds = 2 * (size_t)kWave * cps * sizeof(T);
+  const int ns = pair ? 2 : 1;
+  const long long per = (long long)kWave * ns;
+  const unsigned blocks = (unsigned)((D.nsys + per - 1) / per);
+  const size_t lds = 2 * (size_t)ns * kWave * cps * sizeof(T);
   const dim3 grid(blocks), block(kWave * cps);
   const T *f = static_cast<const T *>(F);
-#define TRID_CC(RR, II)                                                                     \
-  hipLaunchKernelGGL((k_cc_sweep<T, INC, RR, II>), grid, block, lds, st, f, d, u, D, cps_log2, \
-                     xcd)
+#define TRID_CC(RR, II, NN)                                                                \
+  hipLaunchKernelGGL((k_cc_sweep<T, INC, RR, II, NN>), grid, block, lds, st, f, d, u, D,   \
+                     cps_log2, xcd)
+#define TRID_CC_R(II, NN)      \
+  if (R == 8) TRID_CC(8, II, NN);       \
+  else if (R == 16) TRID_CC(16, II, NN); \
+  else TRID_CC(32, II, NN);
+  // pairs: chunk lengths 8 and 16 only (32 rows of two systems exceed the
+  // register budget of 1024-thread work-groups)
   if (idx32) {
-    if (R == 8) TRID_CC(8, uint32_t);
-    else if (R == 16) TRID_CC(16, uint32_t);
-    else TRID_CC(32, uint32_t);
+    if (pair) {
+      if (R == 8) TRID_CC(8, uint32_t, 2);
+      else TRID_CC(16, uint32_t, 2);
+    } else {
+      TRID_CC_R(uint32_t, 1)
+    }
   } else {
-    if (R == 8) TRID_CC(8, long long);
-    else if (R == 16) TRID_CC(16, long long);
-    else TRID_CC(32, long long);
+    if (pair) {
+      if (R == 8) TRID_CC(8, long long, 2);
+      else TRID_CC(16, long long, 2);
+    } else {
+      TRID_CC_R(long long, 1)
+    }
   }
+#undef TRID_CC_R
 #undef TRID_CC
   return hipGetLastError();
 }
 
 template <typename T, int INC>
 hipError_t launch_cc(const tridAdiPlan_st *p, int dir, T *d, T *u, hipStream_t st) {
-  return launch_cc_t<T, INC>(p->cc_R[dir], p->cc_cps_log2[dir], p->cc_xcd[dir], p->cc_F[dir],
-                             p->desc[dir], p->cfg[dir].idx32, d, u, st);
+  const int pair = p->cc_pair[dir] && p->cc_R[dir] <= 16 && cc_pair_ok<T>(p->desc[dir], d, u, INC);
+  return launch_cc_t<T, INC>(p->cc_R[dir], p->cc_cps_log2[dir], p->cc_xcd[dir], pair,
+                             p->cc_F[dir], p->desc[dir], p->cfg[dir].idx32, d, u, st);
 }
 
 // Stage 4 = fused RHS + x sweep (plans with fused_R > 0 only).
@@ -498,15 +525,16 @@ void tune_plan_t(tridAdiPlan_st *p, T *u, T *du) {
   // length that fits, in both work-group orders
   for (int dir = 1; dir < 3; dir++) {
     struct Choice {
-      int cc, R, cl, xcd;
+      int cc, R, cl, xcd, pair;
       PartCfg cfg;
       void *F;
     };
     std::vector<Choice> ch;
     for (PartCfg c : candidates(1, n[dir], 0, sizeof(T), true, p->desc[dir].nsys)) {
       c.idx32 = p->cfg[dir].idx32;
-      ch.push_back({0, 0, 0, 0, c, nullptr});
+      ch.push_back({0, 0, 0, 0, 0, c, nullptr});
     }
+    const bool pair_ok = cc_pair_ok<T>(p->desc[dir], du, u, dir == 2);
     if (p->cc_F[dir]) (void)hipFree(p->cc_F[dir]);
     p->cc_F[dir] = nullptr;
     p->cc_R[dir] = 0;
@@ -517,8 +545,10 @@ void tune_plan_t(tridAdiPlan_st *p, T *u, T *du) {
       if (!cc_rows_ok(n[dir], R, &cl)) continue;
       if (cc_build<T>(p, dir, R, &F, &cl) != TRID_STATUS_SUCCESS) continue;
       tables.push_back(F);
-      ch.push_back({1, R, cl, 0, p->cfg[dir], F});
-      ch.push_back({1, R, cl, 1, p->cfg[dir], F});
+      for (int pr = 0; pr <= (pair_ok && R <= 16 ? 1 : 0); pr++) {
+        ch.push_back({1, R, cl, 0, pr, p->cfg[dir], F});
+        ch.push_back({1, R, cl, 1, pr, p->cfg[dir], F});
+      }
     }
     if (ch.empty()) continue;
     const PartCfg keep = p->cfg[dir];
@@ -527,6 +557,7 @@ void tune_plan_t(tridAdiPlan_st *p, T *u, T *du) {
         p->cc_R[dir] = c.R;
         p->cc_cps_log2[dir] = c.cl;
         p->cc_xcd[dir] = c.xcd;
+        p->cc_pair[dir] = c.pair;
         p->cc_F[dir] = c.F;
         p->cfg[dir] = keep;
       } else {
@@ -555,10 +586,11 @@ void tune_plan_t(tridAdiPlan_st *p, T *u, T *du) {
     const int i = pick_in_step((int)ch.size(), prefix, cand, suffix, st, &tm);
     if (tune_log())
       for (size_t c = 0; c < ch.size(); c++)
-        fprintf(stderr, "[tridhip tune] %s sweep %s%s: %.4f ms\n", dir == 1 ? "y" : "z",
+        fprintf(stderr, "[tridhip tune] %s sweep %s%s%s: %.4f ms\n", dir == 1 ? "y" : "z",
                 ch[c].cc ? ("cc<R" + std::to_string(ch[c].R) + ">").c_str()
                          : cfg_name(ch[c].cfg, sizeof(T)).c_str(),
-                ch[c].cc && ch[c].xcd ? ",xcd" : "", tm[c]);
+                ch[c].cc && ch[c].xcd ? ",xcd" : "", ch[c].cc && ch[c].pair ? ",pair" : "",
+                tm[c]);
     (void)hipStreamSynchronize(st);
     apply(ch[i >= 0 ? i : 0]);
     for (void *F : tables)
@@ -596,7 +628,8 @@ std::string plan_text(const tridAdiPlan_st *p) {
   auto sweep_name = [&](int dir) {
     if (!p->cc_R[dir]) return cfg_name(p->cfg[dir], ts);
     return "cc<R" + std::to_string(p->cc_R[dir]) + ",cps" +
-           std::to_string(1 << p->cc_cps_log2[dir]) + (p->cc_xcd[dir] ? ",xcd>" : ">");
+           std::to_string(1 << p->cc_cps_log2[dir]) + (p->cc_xcd[dir] ? ",xcd" : "") +
+           (p->cc_pair[dir] ? ",pair>" : ">");
   };
   std::string fx = "rhs<";
   if (p->fused_R) fx = "rhs+x(fused,R" + std::to_string(p->fused_R) + ")<";
@@ -685,6 +718,7 @@ EXTERN_C tridStatus_t tridAdiPlanCreate(tridAdiPlan_t *plan, int nx, int ny,
       if (r == TRID_STATUS_SUCCESS) {
         p->cc_R[dir] = R;
         p->cc_cps_log2[dir] = cl;
+        p->cc_pair[dir] = R <= 16;  // used when the arrays allow it (cc_pair_ok)
         p->cc_F[dir] = F;
       }
       break;

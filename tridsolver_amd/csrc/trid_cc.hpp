@@ -148,13 +148,18 @@ __global__ void __launch_bounds__(1024)
 }
 
 // The solve with precomputed factors F (k_cc_setup of the same N, R, cps).
-// blockDim = 64 * cps; system q = 64 * block + lane; wave w = chunk w.
+// blockDim = 64 * cps; wave w = chunk w; lane l of work-group b takes NS
+// adjacent systems q = NS * (64 * b + l) + s, s < NS.  NS = 2 moves both
+// systems' rows as one 16-byte access (the host requires adjacent systems to
+// be adjacent elements of d and u, 16-byte aligned, and an even system
+// count), so a wave's row access covers 128 systems without gaps.
 // Non-Inc: d <- A^{-1} d; Inc: u <- u + A^{-1} d.
-template <typename T, int INC, int R, typename I>
+template <typename T, int INC, int R, typename I, int NS>
 __global__ void __launch_bounds__(1024)
     k_cc_sweep(const T *__restrict__ F, T *__restrict__ Dd, T *__restrict__ U, SysDesc D,
                int cps_log2, int xcd) {
   using Ly = CCLayout<R>;
+  typedef T V2 __attribute__((ext_vector_type(2)));
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *sm = reinterpret_cast<T *>(smem_raw);
   const int tid = threadIdx.x;
@@ -168,84 +173,119 @@ __global__ void __launch_bounds__(1024)
     const unsigned nb8 = gridDim.x & ~7u;
     if (blockIdx.x < nb8) block = (long long)(blockIdx.x & 7u) * (nb8 >> 3) + (blockIdx.x >> 3);
   }
-  const long long q = block * kWave + lane;
-  const bool active = q < D.nsys;
+  const long long q0 = (block * kWave + lane) * NS;
+  const bool active = q0 < D.nsys;
   const int N = D.N;
   I off[5];
-  bool line_in = false;
+  bool line_in[NS];
   if (active) {
-    sys_offsets<I>(D, q, off, &line_in);
+    sys_offsets<I>(D, q0, off, &line_in[0]);
+    if (NS == 2) {
+      I off1[5];
+      sys_offsets<I>(D, q0 + 1, off1, &line_in[NS - 1]);
+    }
   } else {
 #pragma unroll
     for (int t = 0; t < 5; t++) off[t] = 0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) line_in[s] = false;
   }
   const int row0 = k * R;
   const I s3 = (I)D.sstr[3], s4 = (I)D.sstr[4];
-  T d[R];
+  // element r of system s: d[s][r]
+  T d[NS][R];
   {
     I od = off[3] + (I)row0 * s3;
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      d[r] = active && row0 + r < N ? ld_nt(at(Dd, od)) : T(0);
+      const bool ok = active && row0 + r < N;
+      if (NS == 2) {
+        const V2 v = ok ? ld_nt(reinterpret_cast<const V2 *>(at(Dd, od))) : V2{};
+        d[0][r] = v[0];
+        d[NS - 1][r] = v[1];
+      } else {
+        d[0][r] = ok ? ld_nt(at(Dd, od)) : T(0);
+      }
       od += s3;
     }
   }
-  // chunk sweep (d only)
-  {
-    T dp = d[1] * Fk[Ly::R1];
-    d[1] = dp;
+  T ld[NS], fd[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    // chunk sweep (d only)
+    T dp = d[s][1] * Fk[Ly::R1];
+    d[s][1] = dp;
 #pragma unroll
     for (int i = 2; i < R; i++) {
-      dp = (d[i] - Fk[Ly::A + i - 2] * dp) * Fk[Ly::RI + i - 2];
-      d[i] = dp;
+      dp = (d[s][i] - Fk[Ly::A + i - 2] * dp) * Fk[Ly::RI + i - 2];
+      d[s][i] = dp;
     }
-  }
-  const T ld = d[R - 1];
+    ld[s] = d[s][R - 1];
 #pragma unroll
-  for (int i = R - 3; i >= 1; i--) d[i] = d[i] - Fk[Ly::CB + i - 1] * d[i + 1];
-  const T fd = (d[0] - Fk[Ly::C0] * d[1]) * Fk[Ly::R0];
-  // reduced system, d only, through LDS (ping-pong buffers of 64 * cps)
-  const int n = cps * kWave;
-  const int me = k * kWave + lane;
-  int par = 0;
-  sm[me] = fd;
-  __syncthreads();
-  const T fnd = k + 1 < cps ? sm[me + kWave] : T(0);
-  par = 1;
-  T rd = (ld - Fk[Ly::LA] * fd - Fk[Ly::LC] * fnd) * Fk[Ly::INV];
-  int j = 0;
-  for (int s = 1; s < cps; s <<= 1, j++) {
-    T *b = sm + par * n;
-    b[me] = rd;
-    __syncthreads();
-    const T md = k >= s ? b[me - s * kWave] : T(0);
-    const T pd = k + s < cps ? b[me + s * kWave] : T(0);
-    par ^= 1;
-    rd = (rd - Fk[Ly::PCR + 3 * j] * md - Fk[Ly::PCR + 3 * j + 1] * pd) *
-         Fk[Ly::PCR + 3 * j + 2];
+    for (int i = R - 3; i >= 1; i--) d[s][i] = d[s][i] - Fk[Ly::CB + i - 1] * d[s][i + 1];
+    fd[s] = (d[s][0] - Fk[Ly::C0] * d[s][1]) * Fk[Ly::R0];
   }
-  const T xl = rd;
-  T xf;
+  // reduced systems, d only, through LDS (ping-pong buffers of NS * 64 * cps)
+  const int n = NS * cps * kWave;
+  const int me = (k * kWave + lane) * NS;
+  int par = 0;
+#pragma unroll
+  for (int s = 0; s < NS; s++) sm[me + s] = fd[s];
+  __syncthreads();
+  T rd[NS];
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const T fnd = k + 1 < cps ? sm[me + NS * kWave + s] : T(0);
+    rd[s] = (ld[s] - Fk[Ly::LA] * fd[s] - Fk[Ly::LC] * fnd) * Fk[Ly::INV];
+  }
+  par = 1;
+  int j = 0;
+  for (int st = 1; st < cps; st <<= 1, j++) {
+    T *b = sm + par * n;
+#pragma unroll
+    for (int s = 0; s < NS; s++) b[me + s] = rd[s];
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const T md = k >= st ? b[me - st * NS * kWave + s] : T(0);
+      const T pd = k + st < cps ? b[me + st * NS * kWave + s] : T(0);
+      rd[s] = (rd[s] - Fk[Ly::PCR + 3 * j] * md - Fk[Ly::PCR + 3 * j + 1] * pd) *
+              Fk[Ly::PCR + 3 * j + 2];
+    }
+    par ^= 1;
+  }
+  T xf[NS];
   {
     T *b = sm + par * n;
-    b[me] = xl;
+#pragma unroll
+    for (int s = 0; s < NS; s++) b[me + s] = rd[s];
     __syncthreads();
-    const T xm = k > 0 ? b[me - kWave] : T(0);
-    xf = fd - Fk[Ly::FA] * xm - Fk[Ly::FC] * xl;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const T xm = k > 0 ? b[me - NS * kWave + s] : T(0);
+      xf[s] = fd[s] - Fk[Ly::FA] * xm - Fk[Ly::FC] * rd[s];
+    }
   }
-  T y[R];
-  y[0] = xf;
-  y[R - 1] = xl;
+  T y[NS][R];
 #pragma unroll
-  for (int i = 1; i < R - 1; i++) y[i] = d[i] - Fk[Ly::EA + i - 1] * xf - Fk[Ly::EC + i - 1] * xl;
+  for (int s = 0; s < NS; s++) {
+    y[s][0] = xf[s];
+    y[s][R - 1] = rd[s];
+#pragma unroll
+    for (int i = 1; i < R - 1; i++)
+      y[s][i] = d[s][i] - Fk[Ly::EA + i - 1] * xf[s] - Fk[Ly::EC + i - 1] * rd[s];
+  }
   if (!active) return;
-  if (!line_in) {
-    // identity system: the solution is d itself (not yet overwritten)
-    I od = off[3] + (I)row0 * s3;
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-      if (row0 + r < N) y[r] = *at(Dd, od);
-      od += s3;
+  for (int s = 0; s < NS; s++) {
+    if (!line_in[s]) {
+      // identity system: the solution is d itself (not yet overwritten)
+      I od = off[3] + (I)row0 * s3 + (I)s;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        if (row0 + r < N) y[s][r] = *at(Dd, od);
+        od += s3;
+      }
     }
   }
   if (INC) {
@@ -253,8 +293,16 @@ __global__ void __launch_bounds__(1024)
 #pragma unroll
     for (int r = 0; r < R; r++) {
       if (row0 + r < N) {
-        T *pu = at(U, ou);
-        st_nt(pu, ld_nt(pu) + y[r]);
+        if (NS == 2) {
+          V2 *pu = reinterpret_cast<V2 *>(at(U, ou));
+          V2 v = ld_nt(pu);
+          v[0] += y[0][r];
+          v[1] += y[NS - 1][r];
+          st_nt(pu, v);
+        } else {
+          T *pu = at(U, ou);
+          st_nt(pu, ld_nt(pu) + y[0][r]);
+        }
       }
       ou += s4;
     }
@@ -262,7 +310,16 @@ __global__ void __launch_bounds__(1024)
     I od = off[3] + (I)row0 * s3;
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      if (row0 + r < N) st_nt(at(Dd, od), y[r]);
+      if (row0 + r < N) {
+        if (NS == 2) {
+          V2 v;
+          v[0] = y[0][r];
+          v[1] = y[NS - 1][r];
+          st_nt(reinterpret_cast<V2 *>(at(Dd, od)), v);
+        } else {
+          st_nt(at(Dd, od), y[0][r]);
+        }
+      }
       od += s3;
     }
   }
