@@ -30,7 +30,10 @@ sample.  "extras" (same line) holds the other measured workloads:
   vadi_grad 128^3 FP64, 16 steps, checkpoint every 4: loss + gradient vs the
             forward pass;
   block     65,536 systems x 128 block rows, M in {2,3,4,5,8}, FP64, and all
-            five in one mixed-M launch.
+            five in one mixed-M launch;
+  split     systems longer than one work-group (split-line solves): 8
+            systems of 2^22 rows (solvedim 0) and 64 systems of 2^19 rows
+            (solvedim 1), FP64 and FP32.
 """
 from __future__ import annotations
 
@@ -59,7 +62,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
-    ap.add_argument("--extras", default="plain,ragged,vadi,block",
+    ap.add_argument("--extras", default="plain,ragged,vadi,block,split",
                     help="comma-separated subset of plain,ragged,vadi,block")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (profiler calibration)")
@@ -293,6 +296,43 @@ def extra_plain(torch, np, ta, n=256, reps=10):
         del dev
     return {"workload": f"{n}^3, one solve per solvedim, inputs as the plain workload",
             "bytes_model": "5 words per unknown (read a, b, c, d; write d)", **out}
+
+
+def extra_split(torch, np, ta, reps=10):
+    """Split-line solves: 2^25 unknowns in few, very long systems."""
+    lib = ta.lib()
+    from tridsolver_amd.api import _iv
+    out = {"workload": "2^25 unknowns: [2^22 rows x 8] solvedim 0, [64 x 2^19 rows] solvedim 1",
+           "bytes_model": "5 words per unknown (one-pass minimum); the split-line solve "
+                          "moves 9 (a, b, c, d read twice, d written once)"}
+    rng = np.random.default_rng(0)
+    n = 1 << 25
+    a = rng.uniform(-1.0, -0.1, n)
+    c = rng.uniform(-1.0, -0.1, n)
+    b = np.abs(a) + np.abs(c) + rng.uniform(0.5, 1.5, n)
+    d = rng.uniform(-1.0, 1.0, n)
+    for name, dt, ws, fn in (("f64", np.float64, 8, lib.tridDmtsvStridedBatch),
+                             ("f32", np.float32, 4, lib.tridSmtsvStridedBatch)):
+        dev = [torch.from_numpy(x.astype(dt)).to("cuda") for x in (a, b, c, d)]
+        p = [t.data_ptr() for t in dev]
+        for tag, dims, sd in (("sd0_4Mx8", [1 << 22, 8], 0), ("sd1_64x512K", [64, 1 << 19], 1)):
+            dv = _iv(dims)
+
+            def solve():
+                st = fn(p[0], p[1], p[2], p[3], None, 2, sd, dv, dv, None, 0)
+                if st:
+                    raise RuntimeError(ta.last_error())
+            solve()
+            solve()
+            ms = _timed(torch, solve, reps)
+            gbs = 5 * ws * n / (ms * 1e-3) / 1e9
+            out[f"{name}_{tag}"] = {
+                "ms": round(ms, 5), "Gunknowns_s": round(n / (ms * 1e-3) / 1e9, 2),
+                "GB_s_5words": round(gbs, 1), "frac_5words": round(gbs / HBM_PEAK_GBS, 4),
+                "frac_9words": round(gbs * 9 / 5 / HBM_PEAK_GBS, 4),
+                "config": ta.last_config()}
+        del dev
+    return out
 
 
 def extra_ragged(torch, np, ta, reps=10):
@@ -549,7 +589,7 @@ def main():
         del W
         torch.cuda.empty_cache()
         fns = {"plain": extra_plain, "ragged": extra_ragged, "vadi": extra_vadi,
-               "block": extra_block}
+               "block": extra_block, "split": extra_split}
         out["extras"] = {}
         for k in args.extras.split(","):
             k = k.strip()

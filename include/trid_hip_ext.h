@@ -39,9 +39,13 @@ EXTERN_C const char *tridGetLastConfig(void);
  * 1 = partitioned Thomas with wave/LDS reduced-system PCR, 2 = one thread
  * per system with a global-memory forward sweep (the fallback), 3 = kind 1
  * with the XCD-contiguous work-group order (every XCD takes one contiguous
- * range of systems).  rows is the chunk length R of kinds 1 and 3 (4, 8 or
- * 16; 0 = automatic), cps the chunks per system (power of two;
- * 0 = automatic), spb the systems per work-group (0 = automatic). */
+ * range of systems), 4 = split-line solve (every system cut into segments of
+ * rows * cps rows that are reduced, coupled through a reduced system of two
+ * rows per segment and solved as one batch; automatic for systems longer
+ * than 8192 rows).  rows is the chunk length R of kinds 1, 3 and 4 (4, 8 or
+ * 16; 0 = automatic), cps the chunks per system (kind 4: per segment; power
+ * of two; 0 = automatic), spb the systems (kind 4: segments) per work-group
+ * (0 = automatic). */
 EXTERN_C tridStatus_t tridForceConfig(int kind, int rows, int cps, int spb);
 
 /* Diagnostic: y[i] = the solvers' reciprocal of x[i] for n device values

@@ -33,8 +33,19 @@ __global__ void __launch_bounds__((R >= 16 && sizeof(T) == 8) ? 512 : 1024)
                                        block, Out);
 }
 
+// Segment steps of a split-line solve: D's systems are segments of longer
+// systems.  MODE 1: their merged reduced rows go to `so`; MODE 2: solved
+// with the couplings to the neighbouring segments taken from so.rd.
+template <typename T, int MODE, int INC, int ORDER, int R, int VEC, typename I>
+__global__ void __launch_bounds__((R >= 16 && sizeof(T) == 8) ? 512 : 1024)
+    k_seg(CoefArrays<T> coef, T *__restrict__ Dd, T *__restrict__ U, SysDesc D, int cps_log2,
+          int spb_log2, SegOut<T> so) {
+  part_block<T, INC, ORDER, R, VEC, I, CoefArrays<T>, MODE>(
+      coef, Dd, U, D, cps_log2, spb_log2, blockIdx.x, nullptr, so);
+}
+
 struct PartCfg {
-  int kind = 1;  // 1 partitioned, 2 fallback
+  int kind = 1;  // 1 partitioned, 2 fallback, 4 split-line (segments of R*cps rows)
   int R = 8;
   int cps_log2 = 0;
   int spb_log2 = 0;
@@ -114,7 +125,57 @@ hipError_t launch_part(const PartCfg &cfg, const Coef &coef, T *d, T *u,
   }
 }
 
+template <typename T, int MODE, int INC, int ORDER, int R, int VEC>
+hipError_t launch_seg_t(const PartCfg &cfg, const CoefArrays<T> &coef, T *d, T *u,
+                        const SysDesc &D, const SegOut<T> &so, hipStream_t st) {
+  const int cps = 1 << cfg.cps_log2, spb = 1 << cfg.spb_log2;
+  const int threads = cps * spb;
+  const long long blocks = (D.nsys + spb - 1) / spb;
+  size_t lds = 0;
+  // LDS: the merge tree (MODE 1) or the reduced solve (MODE 2) runs in LDS
+  // unless all chunks of a segment are in one wave (ORDER 0, cps <= 64)
+  if (ORDER == 1 || cps > kWave) lds = 2 * 3 * (size_t)cps * (spb + 1) * sizeof(T);
+  if (VEC == 2) lds = (size_t)threads * R * sizeof(T);
+  if (cfg.idx32)
+    hipLaunchKernelGGL((k_seg<T, MODE, INC, ORDER, R, VEC, uint32_t>), dim3((unsigned)blocks),
+                       dim3(threads), lds, st, coef, d, u, D, cfg.cps_log2, cfg.spb_log2, so);
+  else
+    hipLaunchKernelGGL((k_seg<T, MODE, INC, ORDER, R, VEC, long long>), dim3((unsigned)blocks),
+                       dim3(threads), lds, st, coef, d, u, D, cfg.cps_log2, cfg.spb_log2, so);
+  return hipGetLastError();
+}
+
+// Launches segment step MODE (1 reduce, 2 solve) for a configuration of the
+// table.
+template <typename T, int MODE, int INC>
+hipError_t launch_seg(const PartCfg &cfg, const CoefArrays<T> &coef, T *d, T *u,
+                      const SysDesc &D, const SegOut<T> &so, hipStream_t st) {
+#define TRID_SEG(ORD, RR, VV)                                                  \
+  if (cfg.order == ORD && cfg.R == RR && cfg.vec == VV)                        \
+    return launch_seg_t<T, MODE, INC, ORD, RR, VV>(cfg, coef, d, u, D, so, st);
+  TRID_SEG(0, 4, 0)
+  TRID_SEG(0, 8, 0)
+  TRID_SEG(0, 16, 0)
+  TRID_SEG(0, 4, 1)
+  TRID_SEG(0, 8, 1)
+  TRID_SEG(0, 16, 1)
+  TRID_SEG(0, 4, 2)
+  TRID_SEG(0, 8, 2)
+  TRID_SEG(0, 16, 2)
+  TRID_SEG(1, 4, 0)
+  TRID_SEG(1, 8, 0)
+  TRID_SEG(1, 16, 0)
+#undef TRID_SEG
+  return hipErrorInvalidValue;
+}
+
 inline bool cfg_valid(const PartCfg &cfg, int N) {
+  if (cfg.kind == 4) {
+    // split-line: the segment shape must be a valid work-group shape
+    PartCfg s = cfg;
+    s.kind = 1;
+    return cfg_valid(s, cfg.R << cfg.cps_log2);
+  }
   if (cfg.kind != 1) return true;
   const long long cps = 1LL << cfg.cps_log2, spb = 1LL << cfg.spb_log2;
   const long long thr = cps * spb;
@@ -154,6 +215,10 @@ inline std::string cfg_name(const PartCfg &c, size_t tsize) {
   char buf[128];
   if (c.kind == 2) {
     snprintf(buf, sizeof buf, "fallback<%s>", tsize == 8 ? "double" : "float");
+  } else if (c.kind == 4) {
+    snprintf(buf, sizeof buf, "split<%s,o%d,R%d,L%d,spb%d%s>",
+             tsize == 8 ? "double" : "float", c.order, c.R, c.R << c.cps_log2,
+             1 << c.spb_log2, c.vec == 2 ? ",staged" : (c.vec ? ",vec" : ""));
   } else {
     snprintf(buf, sizeof buf, "part<%s,o%d,R%d,cps%d,spb%d%s%s>",
              tsize == 8 ? "double" : "float", c.order, c.R, 1 << c.cps_log2,
@@ -236,6 +301,29 @@ int pick_fastest(int ncand, Prep prep, Run run, hipStream_t st,
   return arg;
 }
 
+// Split-line configuration for systems longer than one work-group holds:
+// segments of L = R * cps rows, from sweeps on MI355X over 2^25 unknowns:
+//   ORDER 0: 64 chunks per segment (L = 512 FP64 / 1024 FP32), 256 threads;
+//   ORDER 1: up to 64 systems side by side and 512 (FP64) / 256 (FP32)
+//            threads -- short segments (L = 64) for wide batches, which beat
+//            longer ones despite the longer reduced system.
+inline PartCfg split_cfg(int order, int vec, size_t tsize, long long nsys) {
+  PartCfg c;
+  c.kind = 4;
+  c.order = order;
+  c.R = tsize == 8 ? 8 : 16;
+  c.vec = order == 0 && vec && (c.R * (int)tsize) % 16 == 0;
+  if (order == 0) {
+    c.cps_log2 = 6;
+    c.spb_log2 = 2;
+  } else {
+    c.spb_log2 = std::min(6, ilog2(std::max(1LL, nsys)));
+    c.cps_log2 = std::max(1, (tsize == 8 ? 9 : 8) - c.spb_log2);
+    if (c.spb_log2 + c.cps_log2 < 6) c.spb_log2 = 6 - c.cps_log2;
+  }
+  return c;
+}
+
 inline PartCfg heuristic(int order, int N, int vec, size_t tsize) {
   // preferred chunk lengths: FP64 8 rows, FP32 16 rows (register budget)
   const int pref[3] = {tsize == 8 ? 8 : 16, tsize == 8 ? 16 : 8, 4};
@@ -253,9 +341,7 @@ inline PartCfg heuristic(int order, int N, int vec, size_t tsize) {
       if (cfg_valid(c, N)) return c;
     }
   }
-  PartCfg f;
-  f.kind = 2;
-  return f;
+  return split_cfg(order, vec, tsize, 1 << 20);
 }
 
 }  // namespace tridhip

@@ -108,6 +108,64 @@ __device__ __forceinline__ Row<T> eliminate_first(const Row<T> &f,
   return r;
 }
 
+// Merges the reduced rows of two adjacent blocks of rows, A = (f, l) and the
+// block B = (fb, lb) just after it, into the reduced rows of A followed by B:
+// A's last and B's first unknown are eliminated (a 2x2 solve of l's and fb's
+// rows), which leaves
+//   f: f.a * y_{-1} + y_fA + f.c * y_lB = f.d
+//   l: l.a * y_fA + y_lB + l.c * y_{+1} = l.d
+// with y_{-1} / y_{+1} the unknowns just outside A / B.  The operation is
+// associative, so a segment's rows can be merged in a tree.
+template <typename T>
+__device__ __forceinline__ void seg_merge(Row<T> &f, Row<T> &l, const Row<T> &fb,
+                                          const Row<T> &lb) {
+  const T idet = rcp(T(1) - l.c * fb.a);
+  const T g = rcp(T(1) - f.c * l.a * idet);
+  const T h = rcp(T(1) - lb.a * fb.c * idet);
+  Row<T> F, L;
+  F.a = f.a * g;
+  F.c = f.c * l.c * fb.c * idet * g;
+  F.d = (f.d - f.c * (l.d - l.c * fb.d) * idet) * g;
+  L.a = lb.a * fb.a * l.a * idet * h;
+  L.c = lb.c * h;
+  L.d = (lb.d - lb.a * (fb.d - fb.a * l.d) * idet) * h;
+  f = F;
+  l = L;
+}
+
+// Output of the segment-reduction mode of part_block (split-line solves of
+// systems longer than one work-group holds).  The work-group's systems are
+// segments of L rows of longer systems; each segment's two merged reduced
+// rows go to rows 2p and 2p+1 of its system's reduced system (b = 1), at
+// element qs * rs_sys + row * rs_row of ra, rb, rc, rd.
+//   segment item q -> (system qs, segment p): p_tail >= 0: the launch holds
+//   only the last (partial) segment p_tail of every system, qs = q;
+//   otherwise ORDER 0: p = q % Pf, qs = q / Pf; ORDER 1: qs = q % nsys,
+//   p = q / nsys.
+template <typename T>
+struct SegOut {
+  T *ra = nullptr, *rb = nullptr, *rc = nullptr, *rd = nullptr;
+  long long rs_row = 0, rs_sys = 0;
+  long long nsys = 0;
+  int Pf = 1, P = 1, p_tail = -1;
+};
+
+// (system, segment) of segment item q of a SegOut launch.
+template <typename T, int ORDER>
+__device__ __forceinline__ void seg_item(const SegOut<T> &so, long long q, long long &qs,
+                                         int &p) {
+  if (so.p_tail >= 0) {
+    qs = q;
+    p = so.p_tail;
+  } else if (ORDER == 0) {
+    p = (int)(q % so.Pf);
+    qs = q / so.Pf;
+  } else {
+    qs = q % so.nsys;
+    p = (int)(q / so.nsys);
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ Row<T> row_from_lane(const Row<T> &v, int src,
                                                 bool valid) {
@@ -540,14 +598,26 @@ struct CoefKappa {
 //   I:       index type of element offsets (uint32_t or long long).
 // blockDim.x == cps * spb, both powers of two.  Non-Inc: d <- A^{-1} d;
 // Inc: u <- u + A^{-1} d.
+// MODE 1 (segment reduction, CoefArrays only): every system of D is a
+// segment of a longer system; instead of solving it, the work-group merges
+// each segment's chunk rows into the segment's two reduced rows -- with the
+// segment's true couplings a (first row) and c (last row) to its neighbours
+// -- and writes them as SegOut describes.  Nothing else is written.
+// MODE 2 (segment solve, CoefArrays only): every system of D is a segment
+// whose neighbours' boundary unknowns are solved (so.rd holds the reduced
+// solution); the couplings to them move to the right-hand side
+// (d_first -= a_first * last_{p-1}, d_last -= c_last * first_{p+1}) and the
+// segment is solved as an independent system.
 template <typename T, int INC, int ORDER, int R, int VEC, typename I,
-          typename Coef>
+          typename Coef, int MODE = 0>
 __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
                                            T *__restrict__ U, const SysDesc &D,
                                            int cps_log2, int spb_log2,
                                            long long block,
-                                           T *__restrict__ Out = nullptr) {
+                                           T *__restrict__ Out = nullptr,
+                                           SegOut<T> so = SegOut<T>()) {
   static_assert(!Coef::kHook || VEC == 0, "hooked coefficients load scalars");
+  static_assert(MODE == 0 || Coef::kArrays, "segment reduction reads coefficient arrays");
   const int tid = threadIdx.x;
   const int cps = 1 << cps_log2;
   const int spb = 1 << spb_log2;
@@ -691,10 +761,97 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
       if (row0 + R == N) c[R - 1] = T(0);
     }
   }
+  if constexpr (MODE == 2) {
+    if (active && (row0 == 0 || row0 + R == N)) {
+      long long qs;
+      int p;
+      seg_item<T, ORDER>(so, q, qs, p);
+      const long long xb = qs * so.rs_sys;
+      if (row0 == 0 && p > 0)
+        d[0] -= *at(coef.A, off[0]) * so.rd[xb + (2LL * p - 1) * so.rs_row];
+      if (row0 + R == N && p + 1 < so.P)
+        d[R - 1] -= *at(coef.C, off[2] + (I)(N - 1) * s2) * so.rd[xb + (2LL * p + 2) * so.rs_row];
+    }
+  }
+  if constexpr (MODE == 1) {
+    // unit couplings at the segment's ends: the merged rows' outer
+    // coefficients are then proportional to the true a and c, which the
+    // writer below multiplies in (zero at the ends of the whole system); a
+    // partial last chunk ends its system, so its zero c stays
+    if (row0 == 0) a[0] = T(1);
+    if ((VEC == 2 || full) && row0 + R == N) c[R - 1] = T(1);
+  }
 
   // ---- solve -----------------------------------------------------------
   Row<T> f, l;
   chunk_reduce<T, R>(a, b, c, d, f, l);
+
+  if constexpr (MODE == 1) {
+    // merge the segment's chunk rows in a tree; chunk 0 ends with the
+    // segment's rows
+    if (ORDER == 0 && cps <= kWave) {
+      const int lane = tid & (kWave - 1);
+      for (int s = 1; s < cps; s <<= 1) {
+        const int src = (lane + s) & (kWave - 1);
+        Row<T> fb, lb;
+        fb.a = lane_read(f.a, src);
+        fb.c = lane_read(f.c, src);
+        fb.d = lane_read(f.d, src);
+        lb.a = lane_read(l.a, src);
+        lb.c = lane_read(l.c, src);
+        lb.d = lane_read(l.d, src);
+        if ((k & (2 * s - 1)) == 0) seg_merge<T>(f, l, fb, lb);
+      }
+    } else {
+      // entry (chunk k, slot) at k * spb + slot of six arrays of cps * spb
+      // values; at level s chunk k (k % 2s == 0) reads entry k + s, which
+      // was last written at level s/2, and rewrites its own
+      T *sm = reinterpret_cast<T *>(smem_raw);
+      const int n = cps * spb, e = k * spb + slot;
+      auto put = [&]() {
+        sm[e] = f.a;
+        sm[n + e] = f.c;
+        sm[2 * n + e] = f.d;
+        sm[3 * n + e] = l.a;
+        sm[4 * n + e] = l.c;
+        sm[5 * n + e] = l.d;
+      };
+      put();
+      for (int s = 1; s < cps; s <<= 1) {
+        __syncthreads();
+        if ((k & (2 * s - 1)) == 0) {
+          const int o = (k + s) * spb + slot;
+          Row<T> fb, lb;
+          fb.a = sm[o];
+          fb.c = sm[n + o];
+          fb.d = sm[2 * n + o];
+          lb.a = sm[3 * n + o];
+          lb.c = sm[4 * n + o];
+          lb.d = sm[5 * n + o];
+          seg_merge<T>(f, l, fb, lb);
+          put();
+        }
+      }
+    }
+    if (k == 0 && active) {
+      long long qs;
+      int p;
+      seg_item<T, ORDER>(so, q, qs, p);
+      const T as = p > 0 ? *at(coef.A, off[0]) : T(0);
+      const T ce = p + 1 < so.P ? *at(coef.C, off[2] + (I)(N - 1) * s2) : T(0);
+      const long long i0 = qs * so.rs_sys + (long long)(2 * p) * so.rs_row;
+      const long long i1 = i0 + so.rs_row;
+      so.ra[i0] = f.a * as;
+      so.rb[i0] = T(1);
+      so.rc[i0] = f.c;
+      so.rd[i0] = f.d;
+      so.ra[i1] = l.a;
+      so.rb[i1] = T(1);
+      so.rc[i1] = l.c * ce;
+      so.rd[i1] = l.d;
+    }
+    return;
+  }
 
   T xf, xl;
   if (ORDER == 0 && cps <= kWave) {

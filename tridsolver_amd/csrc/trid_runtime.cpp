@@ -124,7 +124,7 @@ EXTERN_C const char *tridGetStatusString(tridStatus_t s) {
 }
 
 EXTERN_C tridStatus_t tridForceConfig(int kind, int rows, int cps, int spb) {
-  if (kind < 0 || kind > 3) return fail(TRID_STATUS_INVALID_VALUE, "tridForceConfig: kind");
+  if (kind < 0 || kind > 4) return fail(TRID_STATUS_INVALID_VALUE, "tridForceConfig: kind");
   if (rows != 0 && rows != 4 && rows != 8 && rows != 16)
     return fail(TRID_STATUS_INVALID_VALUE, "tridForceConfig: rows must be 0, 4, 8 or 16");
   auto pow2 = [](int v) { return v == 0 || (v > 0 && (v & (v - 1)) == 0); };

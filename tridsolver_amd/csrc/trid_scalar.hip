@@ -8,7 +8,9 @@
 // chosen by the layout class:
 //   solvedim 0  (unit stride along the system)  -> ORDER 0, chunks in lanes
 //   solvedim >= 1 (systems adjacent in memory)  -> ORDER 1, systems in lanes
-// and a one-thread-per-system fallback for shapes outside its limits.
+// Systems longer than one work-group holds (8192 rows) take the split-line
+// path below, which runs the same kernels on segments of the systems; a
+// one-thread-per-system Thomas kernel remains as a forced configuration.
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
@@ -103,6 +105,160 @@ hipError_t launch_fallback(const T *a, const T *b, const T *c, T *d, T *u,
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// split-line solves: systems longer than one work-group holds
+//
+// Every system is cut into P segments of L = R * cps rows (the last one may
+// be shorter).  The solve takes three steps, each a batched launch over all
+// segments of all systems:
+//   1. segment reduction (k_seg MODE 1): each work-group reduces its
+//      segments (partitioned Thomas on the chunks, then a merge tree over the
+//      chunks' rows) to the segment's two rows in its first and last
+//      unknowns, coupled to the neighbouring segments -- the reduced system
+//      of 2P rows per system;
+//   2. the reduced systems are solved by the partitioned kernel (or, when
+//      2P is itself too long, by a split-line solve of their own);
+//   3. segment solve (k_seg MODE 2): every segment is solved as an
+//      independent system, its couplings to the neighbouring segments moved
+//      to the right-hand side with their solved boundary unknowns.
+// d is only read until step 3 writes the solution (Inc: never written).
+// Global traffic: a, b, c, d read twice and the solution written once (9
+// values per unknown against 5 for one-pass solves), plus 8 reduced values
+// per segment.  The reference solves the same reduced structure across MPI
+// ranks (scalar/src/cpu/trid_mpi_cpu.hpp:139-328: modified Thomas per rank,
+// reduced system of two rows per rank, then a backward pass per rank); here
+// the ranks are segments of one GPU's batch.
+
+inline long long round4(long long v) { return (v + 3) & ~3LL; }
+
+// Workspace (elements) of a split-line solve, including its recursive
+// reduced solves.
+inline long long split_ws_elems(const PartCfg &cfg, int N, long long nsys, size_t tsize) {
+  const long long L = (long long)cfg.R << cfg.cps_log2;
+  const long long P = (N + L - 1) / L;
+  const long long n2 = round4(2 * P * nsys);
+  long long need = 4 * n2;
+  if (heuristic(cfg.order, (int)(2 * P), 1, tsize).kind != 1)
+    need += split_ws_elems(split_cfg(cfg.order, 1, tsize, nsys), (int)(2 * P), nsys, tsize);
+  return need;
+}
+
+template <typename T, int INC>
+hipError_t split_solve(const PartCfg &cfg, const T *a, const T *b, const T *c, T *d,
+                       T *u, const SysDesc &D, hipStream_t st, T *ws) {
+  const int order = cfg.order;
+  const int L = cfg.R << cfg.cps_log2;
+  const int N = D.N;
+  const int P = (N + L - 1) / L, Pf = N / L;
+  const int Nt = N - Pf * L;
+  const long long ns = D.nsys;
+  const long long n2 = round4(2LL * P * ns);
+  T *ra = ws, *rb = ws + n2, *rc = ws + 2 * n2, *rd = ws + 3 * n2;
+  T *next = ws + 4 * n2;
+
+  SegOut<T> so;
+  so.ra = ra;
+  so.rb = rb;
+  so.rc = rc;
+  so.rd = rd;
+  so.nsys = ns;
+  so.P = P;
+  so.Pf = Pf > 0 ? Pf : 1;
+  so.rs_row = order == 0 ? 1 : ns;
+  so.rs_sys = order == 0 ? 2LL * P : 1;
+
+  // the full segments as one batch: the segment index is one more non-solve
+  // dimension (fastest for ORDER 0, so that a system's segments are adjacent
+  // work items; slowest for ORDER 1, which keeps systems in adjacent lanes)
+  SysDesc Ds = D;
+  Ds.N = L;
+  if (Pf > 0) {
+    const int slot = order == 0 ? 0 : D.nother;
+    for (int k = D.nother; k > slot; k--) {
+      Ds.osize[k] = D.osize[k - 1];
+      for (int t = 0; t < 5; t++) Ds.ostr[t][k] = D.ostr[t][k - 1];
+    }
+    Ds.osize[slot] = Pf;
+    for (int t = 0; t < 5; t++) Ds.ostr[t][slot] = (long long)L * D.sstr[t];
+    Ds.nother = D.nother + 1;
+    Ds.nsys = ns * Pf;
+    Ds.small_index = Ds.nsys < (1LL << 31);
+  }
+  PartCfg cs = cfg;
+  cs.kind = 1;
+  cs.xcd = 0;
+  if (cs.vec == 2) cs.vec = 1;
+  // wave-staged access: the segments of one wave must be adjacent in memory
+  if (cs.vec == 1 && order == 0 && (1 << cs.cps_log2) <= kWave &&
+      Pf % (kWave >> cs.cps_log2) == 0)
+    cs.vec = 2;
+
+  // the last, shorter segment of every system (rows Pf*L .. N-1)
+  SysDesc Dt = D;
+  Dt.N = Nt;
+  PartCfg ct;
+  const long long sh[5] = {(long long)Pf * L * D.sstr[0], (long long)Pf * L * D.sstr[1],
+                           (long long)Pf * L * D.sstr[2], (long long)Pf * L * D.sstr[3],
+                           (long long)Pf * L * D.sstr[4]};
+  if (Nt > 0) {
+    ct = make_cfg(order, cfg.R, Nt, cfg.vec ? 1 : 0, 1 << (cfg.cps_log2 + cfg.spb_log2));
+    ct.idx32 = cfg.idx32;
+    if (!cfg_valid(ct, Nt)) return hipErrorInvalidValue;
+  }
+
+  SegOut<T> sot = so;  // the tail launch
+  sot.p_tail = Pf;
+
+  hipError_t e;
+  // 1. segment reduction
+  if (Pf > 0) {
+    e = launch_seg<T, 1, 0>(cs, CoefArrays<T>{a, b, c}, d, nullptr, Ds, so, st);
+    if (e != hipSuccess) return e;
+  }
+  if (Nt > 0) {
+    e = launch_seg<T, 1, 0>(ct, CoefArrays<T>{a + sh[0], b + sh[1], c + sh[2]}, d + sh[3],
+                            nullptr, Dt, sot, st);
+    if (e != hipSuccess) return e;
+  }
+
+  // 2. reduced systems: 2P rows each, laid out like the original order
+  SysDesc Dr;
+  memset(&Dr, 0, sizeof Dr);
+  Dr.N = 2 * P;
+  Dr.nsys = ns;
+  Dr.nother = 1;
+  Dr.osize[0] = (int)ns;
+  for (int t = 0; t < 5; t++) {
+    Dr.sstr[t] = so.rs_row;
+    Dr.ostr[t][0] = so.rs_sys;
+  }
+  Dr.small_index = 1;
+  const int ridx32 = (unsigned long long)n2 * sizeof(T) < (1ULL << 32);
+  // (16-byte loads along the reduced rows for ORDER 0: the workspace arrays
+  // are 16-byte aligned and each system's 2P rows start at a multiple of 2P)
+  const int rvec = order == 0 && (2 * P * (int)sizeof(T)) % 16 == 0;
+  PartCfg cr = heuristic(order, 2 * P, rvec, sizeof(T));
+  if (cr.kind == 1) {
+    if (stage_ok(cr, order == 0, ns, 2 * P)) cr.vec = 2;
+    cr.idx32 = ridx32;
+    e = launch_part<T, 0>(cr, CoefArrays<T>{ra, rb, rc}, rd, nullptr, Dr, st);
+  } else {
+    cr = split_cfg(order, rvec, sizeof(T), ns);
+    cr.idx32 = ridx32;
+    e = split_solve<T, 0>(cr, ra, rb, rc, rd, nullptr, Dr, st, next);
+  }
+  if (e != hipSuccess) return e;
+
+  // 3. independent segments, couplings from the reduced solution
+  if (Pf > 0) {
+    e = launch_seg<T, 2, INC>(cs, CoefArrays<T>{a, b, c}, d, u, Ds, so, st);
+    if (e != hipSuccess) return e;
+  }
+  if (Nt > 0)
+    e = launch_seg<T, 2, INC>(ct, CoefArrays<T>{a + sh[0], b + sh[1], c + sh[2]}, d + sh[3],
+                              INC ? u + sh[4] : nullptr, Dt, sot, st);
+  return e;
+}
 
 std::mutex g_tune_mu;
 // tuned configuration per (element size, Inc, order, N, systems, layout,
@@ -130,6 +286,11 @@ tridStatus_t run_cfg(const PartCfg &cfg, const T *a, const T *b, const T *c,
     T *ws = static_cast<T *>(workspace(bytes));
     if (!ws) return fail(TRID_STATUS_ALLOC_FAILED, "workspace allocation failed");
     e = launch_fallback<T, INC>(a, b, c, d, u, D, st, ws);
+  } else if (cfg.kind == 4) {
+    const long long el = split_ws_elems(cfg, D.N, D.nsys, sizeof(T));
+    T *ws = static_cast<T *>(workspace((size_t)el * sizeof(T)));
+    if (!ws) return fail(TRID_STATUS_ALLOC_FAILED, "workspace allocation failed");
+    e = split_solve<T, INC>(cfg, a, b, c, d, u, D, st, ws);
   } else {
     e = launch_part<T, INC>(cfg, CoefArrays<T>{a, b, c}, d, u, D, st);
   }
@@ -249,6 +410,20 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
   ForcedConfig fc = forced_config();
   if (fc.kind == 2) {
     cfg.kind = 2;
+  } else if (fc.kind == 4) {
+    cfg = split_cfg(order, vec, sizeof(T), D.nsys);
+    if (fc.rows) cfg.R = fc.rows;
+    if (fc.cps) {
+      // keep the work-group size of the default shape
+      const int thr_log2 = cfg.cps_log2 + cfg.spb_log2;
+      cfg.cps_log2 = ilog2(fc.cps);
+      cfg.spb_log2 = std::max(0, thr_log2 - cfg.cps_log2);
+      if (order == 1) cfg.spb_log2 = std::min(cfg.spb_log2, 6);
+    }
+    if (fc.spb) cfg.spb_log2 = ilog2(fc.spb);
+    cfg.vec = vec && (cfg.R * (int)sizeof(T)) % 16 == 0;
+    if (!cfg_valid(cfg, D.N))
+      return fail(TRID_STATUS_INVALID_VALUE, "forced configuration does not fit this shape");
   } else if (fc.kind == 1 || fc.kind == 3) {
     int R = fc.rows ? fc.rows : (sizeof(T) == 8 ? 8 : 16);
     cfg = make_cfg(order, R, D.N, vec && (R * (int)sizeof(T)) % 16 == 0,
@@ -283,6 +458,10 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
     cfg = heuristic(order, D.N, vec, sizeof(T));
     if (stage_ok(cfg, dense, D.nsys, D.N)) cfg.vec = 2;
   }
+  // systems longer than one work-group holds: split-line solve
+  if (cfg.kind == 4 && fc.kind != 4) cfg = split_cfg(order, vec, sizeof(T), D.nsys);
+  if (cfg.kind == 4 && D.nsys > (1LL << 30))
+    return fail(TRID_STATUS_INVALID_VALUE, "split-line solve: too many systems");
   cfg.idx32 = idx32;
   set_last_config(cfg_name(cfg, sizeof(T)));
   tridStatus_t s = run_cfg<T, INC>(cfg, a, b, c, d, u, D, st);
