@@ -10,6 +10,9 @@ Fixtures written (inputs and expected outputs only -- data, not code):
       final u of the reference's gold ADI (apps/adi/src/adi_orig.c, built
       unmodified from source), raw little-endian floats, x fastest; its own
       initial condition (1 on the boundary, 0 inside) and lambda = 1.
+  adi_ref_{f64,f32}_<nx>x<ny>x<nz>_it<n>_corners.txt
+      the corner tables the same run prints (apps/adi/src/print_array.c),
+      i.e. its standard output between the timing line and "Done.".
   blk_ref_diag_M<m>.npz
       inputs and output of the reference's CPU block solver
       (block/src/blktrid_cpu.cpp dbtsv_cpu, built unmodified) on block
@@ -47,11 +50,16 @@ def adi_fixtures(meta: dict) -> None:
             with tempfile.TemporaryDirectory() as td:
                 dst = os.path.join(td, exe)
                 shutil.copy(os.path.join(REF, exe), dst)
-                subprocess.check_call([dst, "-nx", str(nx), "-ny", str(ny), "-nz", str(nz),
-                                       "-iter", str(it), "-prof", "0"],
-                                      stdout=subprocess.DEVNULL)
+                out = subprocess.run([dst, "-nx", str(nx), "-ny", str(ny), "-nz", str(nz),
+                                      "-iter", str(it), "-prof", "0"],
+                                     stdout=subprocess.PIPE, check=True, text=True).stdout
                 name = f"adi_ref_{tag}_{nx}x{ny}x{nz}_it{it}.bin"
                 shutil.copy(dst + ".dat", os.path.join(HERE, name))
+                lines = out.splitlines(keepends=True)
+                first = next(i for i, ln in enumerate(lines) if ln.startswith("k = "))
+                last = next(i for i, ln in enumerate(lines) if ln.startswith("Done."))
+                with open(os.path.join(HERE, name[:-4] + "_corners.txt"), "w") as f:
+                    f.write("".join(lines[first:last]))
                 meta[name] = {"source": "apps/adi/src/adi_orig.c", "nx": nx, "ny": ny,
                               "nz": nz, "iter": it, "dtype": tag, "lambda": 1.0}
 
