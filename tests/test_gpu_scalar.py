@@ -158,6 +158,20 @@ def test_every_kernel_configuration(cfg, sd, dtype):
         ta.force_config(0, 0, 0, 0)
 
 
+@pytest.mark.parametrize("R", [4, 8, 16])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_staged_multiwave_systems(R, dtype):
+    """Wave-staged access for systems longer than 64 chunks (each wave a
+    contiguous span of one system) with the two-level reduced solve."""
+    ta.force_config(1, R, 0, 0)
+    try:
+        for dims, inc in (([64 * R * 2, 5], False), ([64 * R * 4, 3], True), ([64 * R * 8, 2], False)):
+            run_case(dims, 0, dtype, inc, seed=dims[0])
+            assert "staged" in ta.last_config(), ta.last_config()
+    finally:
+        ta.force_config(0, 0, 0, 0)
+
+
 @pytest.mark.parametrize("N", [8192, 20000])
 def test_long_systems(N):
     for dtype in (np.float64, np.float32):

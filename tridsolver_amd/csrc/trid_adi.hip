@@ -689,7 +689,7 @@ EXTERN_C tridStatus_t tridAdiPlanCreate(tridAdiPlan_t *plan, int nx, int ny,
       PartCfg cu = heuristic(order, n[dir], 0, ts);
       cu.idx32 = idx32;
       p->cfg_x_unaligned = cu;
-      if (stage_ok(c, true, p->desc[0].nsys, nx)) c.vec = 2;
+      if (stage_ok(c, true, p->desc[0].nsys, nx)) c.vec = staged_vec(c);
     }
     c.idx32 = idx32;
     p->cfg[dir] = c;

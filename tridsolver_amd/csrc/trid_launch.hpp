@@ -76,6 +76,8 @@ hipError_t launch_part_t(const PartCfg &cfg, const Coef &coef, T *d, T *u,
   // ORDER 1 with cps <= 64, reduced_lds_w (6 * cps * (spb + 1))
   if (ORDER == 1 || cps > kWave) lds = 2 * 3 * (size_t)cps * (spb + 1) * sizeof(T);
   if (VEC == 2) lds = (size_t)threads * R * sizeof(T);  // wave staging buffers
+  // staging buffers, then 8 values per wave of segment rows (reduced_segs)
+  if (VEC == 3) lds = (size_t)threads * R * sizeof(T) + 8 * (size_t)(threads / kWave) * sizeof(T);
   if (cfg.idx32)
     hipLaunchKernelGGL((k_part<T, INC, ORDER, R, VEC, uint32_t, Coef>),
                        dim3((unsigned)blocks), dim3(threads), lds, st, coef, d,
@@ -117,6 +119,9 @@ hipError_t launch_part(const PartCfg &cfg, const Coef &coef, T *d, T *u,
     TRID_CASE(0, 4, 2)
     TRID_CASE(0, 8, 2)
     TRID_CASE(0, 16, 2)
+    TRID_CASE(0, 4, 3)
+    TRID_CASE(0, 8, 3)
+    TRID_CASE(0, 16, 3)
     TRID_CASE(1, 4, 0)
     TRID_CASE(1, 8, 0)
     TRID_CASE(1, 16, 0)
@@ -201,15 +206,19 @@ inline PartCfg make_cfg(int order, int R, int N, int vec, int threads) {
   return c;
 }
 
-// Wave-staged access (VEC 2) needs dense arrays (system q at q * N), whole
-// chunks (cps * R == N), all chunks of a system in one wave, and whole waves
-// of systems (nsys a multiple of the systems per wave).
+// Wave-staged access (VEC 2 / 3) needs dense arrays (system q at q * N) and
+// whole chunks (cps * R == N); with cps < 64 also whole waves of systems
+// (nsys a multiple of the systems per wave).
 inline bool stage_ok(const PartCfg &c, bool dense, long long nsys, int N) {
   if (c.kind != 1 || c.order != 0 || c.vec != 1 || !dense) return false;
   const long long cps = 1LL << c.cps_log2;
-  if (cps > kWave || cps * c.R != N) return false;
-  return nsys % (kWave / cps) == 0;
+  if (cps * c.R != N) return false;
+  return cps >= kWave || nsys % (kWave / cps) == 0;
 }
+
+// The staged variant of a configuration that stage_ok accepts: VEC 2 with
+// all chunks of a system in one wave, VEC 3 for systems over several waves.
+inline int staged_vec(const PartCfg &c) { return (1 << c.cps_log2) > kWave ? 3 : 2; }
 
 inline std::string cfg_name(const PartCfg &c, size_t tsize) {
   char buf[128];
@@ -218,11 +227,11 @@ inline std::string cfg_name(const PartCfg &c, size_t tsize) {
   } else if (c.kind == 4) {
     snprintf(buf, sizeof buf, "split<%s,o%d,R%d,L%d,spb%d%s>",
              tsize == 8 ? "double" : "float", c.order, c.R, c.R << c.cps_log2,
-             1 << c.spb_log2, c.vec == 2 ? ",staged" : (c.vec ? ",vec" : ""));
+             1 << c.spb_log2, c.vec >= 2 ? ",staged" : (c.vec ? ",vec" : ""));
   } else {
     snprintf(buf, sizeof buf, "part<%s,o%d,R%d,cps%d,spb%d%s%s>",
              tsize == 8 ? "double" : "float", c.order, c.R, 1 << c.cps_log2,
-             1 << c.spb_log2, c.vec == 2 ? ",staged" : (c.vec ? ",vec" : ""),
+             1 << c.spb_log2, c.vec >= 2 ? ",staged" : (c.vec ? ",vec" : ""),
              c.xcd ? ",xcd" : "");
   }
   return buf;
@@ -254,7 +263,7 @@ inline std::vector<PartCfg> candidates(int order, int N, int vec, size_t tsize,
       if (!cfg_valid(c, N)) continue;
       add_both(c);
       if (stage_ok(c, dense, nsys, N)) {
-        c.vec = 2;
+        c.vec = staged_vec(c);
         add_both(c);
       }
     }

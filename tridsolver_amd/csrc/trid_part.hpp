@@ -291,6 +291,81 @@ __device__ __forceinline__ void reduced_lds_w(const Row<T> &f, const Row<T> &l, 
   xl = sm[n + me];
 }
 
+// In-wave PCR of an n-row tridiagonal system (n <= 64) held one row per lane
+// in lanes 0..n-1 (rows of other lanes are ignored); returns lane j's x_j.
+template <typename T>
+__device__ __forceinline__ T pcr_lanes(Row<T> r, int n, int lane) {
+  for (int s = 1; s < n; s <<= 1) {
+    Row<T> m = row_from_lane(r, lane - s, lane >= s && lane < n);
+    Row<T> p = row_from_lane(r, lane + s, lane + s < n);
+    r = pcr_update(r, m, p);
+  }
+  return r.d;
+}
+
+// Reduced solve for systems whose cps > 64 chunks sit in consecutive lanes
+// of W = cps / 64 whole waves (ORDER 0 and the ragged kernel), in two
+// levels with two barriers (against one per PCR level in reduced_lds):
+//   1. each wave merges its 64 chunks' rows in a tree (seg_merge) into the
+//      two rows of its wave segment, which go to LDS;
+//   2. the first wave of every system solves the system's 2W segment rows
+//      (unknowns: first and last of every wave segment) by in-wave PCR;
+//   3. every wave moves the couplings to its neighbouring segments' solved
+//      unknowns into its first and last chunk rows and solves its 64 chunks
+//      with reduced_wave.
+// LDS: 8 * W values per system slot at sm + slot * 8W.  All threads call it.
+template <typename T>
+__device__ __forceinline__ void reduced_segs(const Row<T> &f, const Row<T> &l, int k, int cps,
+                                             int slot, T *sm, T &xf, T &xl) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int W = cps >> 6;
+  const int w = k >> 6;
+  Row<T> F = f, L = l;
+  for (int s = 1; s < kWave; s <<= 1) {
+    const int src = (lane + s) & (kWave - 1);
+    Row<T> fb, lb;
+    fb.a = lane_read(F.a, src);
+    fb.c = lane_read(F.c, src);
+    fb.d = lane_read(F.d, src);
+    lb.a = lane_read(L.a, src);
+    lb.c = lane_read(L.c, src);
+    lb.d = lane_read(L.d, src);
+    if ((lane & (2 * s - 1)) == 0) seg_merge<T>(F, L, fb, lb);
+  }
+  T *rows = sm + slot * (8 * W);  // 2W rows of (a, c, d), then 2W unknowns
+  T *xs = rows + 6 * W;
+  if (lane == 0) {
+    rows[6 * w] = F.a;
+    rows[6 * w + 1] = F.c;
+    rows[6 * w + 2] = F.d;
+    rows[6 * w + 3] = L.a;
+    rows[6 * w + 4] = L.c;
+    rows[6 * w + 5] = L.d;
+  }
+  __syncthreads();
+  if (w == 0) {
+    const int n = 2 * W;
+    Row<T> r;
+    const int j = lane < n ? lane : 0;
+    r.a = rows[3 * j];
+    r.c = rows[3 * j + 1];
+    r.d = rows[3 * j + 2];
+    const T x = pcr_lanes<T>(r, n, lane);
+    if (lane < n) xs[lane] = x;
+  }
+  __syncthreads();
+  Row<T> f2 = f, l2 = l;
+  if (lane == 0 && w > 0) {
+    f2.d -= f2.a * xs[2 * w - 1];
+    f2.a = T(0);
+  }
+  if (lane == kWave - 1 && w + 1 < W) {
+    l2.d -= l2.c * xs[2 * w + 2];
+    l2.c = T(0);
+  }
+  reduced_wave<T>(f2, l2, lane, kWave, lane, xf, xl);
+}
+
 // Streaming (non-temporal) global accesses: every element of a, b, c, d
 // and u is touched exactly once per solve, so the loads and stores carry the
 // non-temporal hint.  On MI355X this lifts a read-4/write-1 stream from
@@ -594,7 +669,9 @@ struct CoefKappa {
 //            unit stride and 16-byte aligned chunks only; checked on the
 //            host).  2: wave-staged access (ORDER 0, cps <= 64, every
 //            system exactly cps*R long and the systems of a wave adjacent in
-//            memory; checked on the host).
+//            memory; checked on the host).  3: the same for cps > 64 (each
+//            wave a contiguous span of one system), with the two-level
+//            reduced solve (reduced_segs).
 //   I:       index type of element offsets (uint32_t or long long).
 // blockDim.x == cps * spb, both powers of two.  Non-Inc: d <- A^{-1} d;
 // Inc: u <- u + A^{-1} d.
@@ -618,6 +695,7 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
                                            SegOut<T> so = SegOut<T>()) {
   static_assert(!Coef::kHook || VEC == 0, "hooked coefficients load scalars");
   static_assert(MODE == 0 || Coef::kArrays, "segment reduction reads coefficient arrays");
+  constexpr bool STAGED = VEC >= 2;
   const int tid = threadIdx.x;
   const int cps = 1 << cps_log2;
   const int spb = 1 << spb_log2;
@@ -650,19 +728,21 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
   T *stage = nullptr;
   I wave_off[5];
   bool wave_active = false;
-  if (VEC == 2) {
-    // the wave's span starts at the offsets of its lane 0 (chunk 0 of the
-    // wave's first system); the host guarantees that a wave's systems are
-    // either all active or all inactive
+  if (STAGED) {
+    // the wave's span starts at the element of its lane 0 (VEC 2: chunk 0
+    // of the wave's first system; VEC 3: chunk 64 w of the wave's system,
+    // unit stride); the host guarantees that a wave's systems are either
+    // all active or all inactive
     stage = reinterpret_cast<T *>(smem_raw) + (tid >> 6) * (kWave * R);
     wave_active = __builtin_amdgcn_readfirstlane((int)active) != 0;
 #pragma unroll
     for (int t = 0; t < 5; t++) {
+      const I o = VEC == 3 ? off[t] + (I)row0 : off[t];
       if constexpr (sizeof(I) == 4) {
-        wave_off[t] = (I)__builtin_amdgcn_readfirstlane((int)off[t]);
+        wave_off[t] = (I)__builtin_amdgcn_readfirstlane((int)o);
       } else {
-        int lo = __builtin_amdgcn_readfirstlane((int)(off[t] & 0xffffffffLL));
-        int hi = __builtin_amdgcn_readfirstlane((int)(off[t] >> 32));
+        int lo = __builtin_amdgcn_readfirstlane((int)(o & 0xffffffffLL));
+        int hi = __builtin_amdgcn_readfirstlane((int)(o >> 32));
         wave_off[t] = (I)(((long long)hi << 32) | (unsigned)lo);
       }
     }
@@ -690,7 +770,7 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
       c[r] = in ? coef.c : T(0);
     }
   }
-  if (VEC == 2) {
+  if (STAGED) {
     // host guarantees: every chunk of an active wave is full and in range
     const int lane = tid & (kWave - 1);
     if (wave_active) {
@@ -756,7 +836,7 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
   if constexpr (Coef::kArrays) {
     // a_0 and c_{N-1} of every system are never used (the partial-chunk
     // path above already masked them)
-    if (VEC == 2 || full) {
+    if (STAGED || full) {
       if (row0 == 0) a[0] = T(0);
       if (row0 + R == N) c[R - 1] = T(0);
     }
@@ -779,7 +859,7 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
     // writer below multiplies in (zero at the ends of the whole system); a
     // partial last chunk ends its system, so its zero c stays
     if (row0 == 0) a[0] = T(1);
-    if ((VEC == 2 || full) && row0 + R == N) c[R - 1] = T(1);
+    if ((STAGED || full) && row0 + R == N) c[R - 1] = T(1);
   }
 
   // ---- solve -----------------------------------------------------------
@@ -856,6 +936,10 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
   T xf, xl;
   if (ORDER == 0 && cps <= kWave) {
     reduced_wave<T>(f, l, k, cps, tid & (kWave - 1), xf, xl);
+  } else if (ORDER == 0 && VEC == 3) {
+    // segment rows after the wave staging buffers
+    reduced_segs<T>(f, l, k, cps, slot,
+                    reinterpret_cast<T *>(smem_raw) + (size_t)blockDim.x * R, xf, xl);
   } else if (cps <= kWave) {
     reduced_lds_w<T>(f, l, k, cps, slot, spb, reinterpret_cast<T *>(smem_raw), xf, xl);
   } else {
@@ -871,7 +955,7 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
   // receives u + solution
   T *__restrict__ Dst = Out ? Out : Dd;
   T *__restrict__ Udst = Out ? Out : U;
-  if (VEC == 2) {
+  if (STAGED) {
     const int lane = tid & (kWave - 1);
     if (wave_active) {
       if (INC)

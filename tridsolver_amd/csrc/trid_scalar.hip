@@ -239,7 +239,7 @@ hipError_t split_solve(const PartCfg &cfg, const T *a, const T *b, const T *c, T
   const int rvec = order == 0 && (2 * P * (int)sizeof(T)) % 16 == 0;
   PartCfg cr = heuristic(order, 2 * P, rvec, sizeof(T));
   if (cr.kind == 1) {
-    if (stage_ok(cr, order == 0, ns, 2 * P)) cr.vec = 2;
+    if (stage_ok(cr, order == 0, ns, 2 * P)) cr.vec = staged_vec(cr);
     cr.idx32 = ridx32;
     e = launch_part<T, 0>(cr, CoefArrays<T>{ra, rb, rc}, rd, nullptr, Dr, st);
   } else {
@@ -305,7 +305,7 @@ PartCfg autotune(int order, int vec, int idx32, bool dense, const T *a,
       candidates(order, D.N, vec, sizeof(T), dense, D.nsys);
   for (auto &cf : cands) cf.idx32 = idx32;
   PartCfg best = heuristic(order, D.N, vec, sizeof(T));
-  if (stage_ok(best, dense, D.nsys, D.N)) best.vec = 2;
+  if (stage_ok(best, dense, D.nsys, D.N)) best.vec = staged_vec(best);
   best.idx32 = idx32;
   if (cands.size() < 2) return best;
   // the candidates run on the caller's own arrays (memory-channel effects of
@@ -432,7 +432,7 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
     if (fc.spb) cfg.spb_log2 = ilog2(fc.spb);
     if (!cfg_valid(cfg, D.N))
       return fail(TRID_STATUS_INVALID_VALUE, "forced configuration does not fit this shape");
-    if (fc.stage && stage_ok(cfg, dense, D.nsys, D.N)) cfg.vec = 2;
+    if (fc.stage && stage_ok(cfg, dense, D.nsys, D.N)) cfg.vec = staged_vec(cfg);
     cfg.xcd = fc.kind == 3;
   } else if (autotune_mode() == 1) {
     auto key = std::make_tuple((int)sizeof(T), INC, order, D.N, D.nsys,
@@ -456,7 +456,7 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
     }
   } else {
     cfg = heuristic(order, D.N, vec, sizeof(T));
-    if (stage_ok(cfg, dense, D.nsys, D.N)) cfg.vec = 2;
+    if (stage_ok(cfg, dense, D.nsys, D.N)) cfg.vec = staged_vec(cfg);
   }
   // systems longer than one work-group holds: split-line solve
   if (cfg.kind == 4 && fc.kind != 4) cfg = split_cfg(order, vec, sizeof(T), D.nsys);
