@@ -172,6 +172,28 @@ def test_staged_multiwave_systems(R, dtype):
         ta.force_config(0, 0, 0, 0)
 
 
+@pytest.mark.parametrize("R", [4, 8, 16])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_general_staging_any_length(R, dtype):
+    """Wave-staged access for any system length (spans that start and end
+    anywhere, boundary pieces stored element by element): odd lengths,
+    partial last waves, systems over several waves, Inc, and padded arrays
+    with one or more waves per system."""
+    ta.force_config(1, R, 0, 0)
+    try:
+        cases = [([100, 37], None, False), ([77, 9], None, True), ([301, 13], None, False),
+                 ([999, 5], None, True), ([1500, 3], None, False), ([4000, 2], None, True),
+                 ([5, 41], None, False), ([64 * R + 3, 3], [64 * R + 8, 3], False),
+                 ([2000, 3], [2005, 4], True)]
+        for dims, pads, inc in cases:
+            run_case(dims, 0, dtype, inc, pads=pads, seed=dims[0])
+            if 64 * R * (4 if R < 16 else 2) >= dims[0]:  # within the staged kernel's reach
+                assert "gstaged" in ta.last_config() or "staged" in ta.last_config(), \
+                    (dims, ta.last_config())
+    finally:
+        ta.force_config(0, 0, 0, 0)
+
+
 @pytest.mark.parametrize("N", [8192, 20000])
 def test_long_systems(N):
     for dtype in (np.float64, np.float32):

@@ -17,7 +17,7 @@ namespace tridhip {
 
 template <typename T, int INC, int ORDER, int R, int VEC, typename I,
           typename Coef>
-__global__ void __launch_bounds__((R >= 16 && sizeof(T) == 8) ? 512 : 1024)
+__global__ void __launch_bounds__(((R >= 16 && sizeof(T) == 8) || VEC == 4) ? 512 : 1024)
     k_part(Coef coef, T *__restrict__ Dd, T *__restrict__ U, SysDesc D,
            int cps_log2, int spb_log2, int xcd, T *__restrict__ Out) {
   // xcd: consecutive hardware blocks go round-robin to the 8 XCDs (each with
@@ -78,6 +78,11 @@ hipError_t launch_part_t(const PartCfg &cfg, const Coef &coef, T *d, T *u,
   if (VEC == 2) lds = (size_t)threads * R * sizeof(T);  // wave staging buffers
   // staging buffers, then 8 values per wave of segment rows (reduced_segs)
   if (VEC == 3) lds = (size_t)threads * R * sizeof(T) + 8 * (size_t)(threads / kWave) * sizeof(T);
+  // general staging: per wave 64 R + E elements (E = 16 / sizeof(T)), then
+  // the segment rows of systems over several waves
+  if (VEC == 4)
+    lds = (size_t)(threads / kWave) * (kWave * R + 16 / sizeof(T) + (cps > kWave ? 8 : 0)) *
+          sizeof(T);
   if (cfg.idx32)
     hipLaunchKernelGGL((k_part<T, INC, ORDER, R, VEC, uint32_t, Coef>),
                        dim3((unsigned)blocks), dim3(threads), lds, st, coef, d,
@@ -122,6 +127,11 @@ hipError_t launch_part(const PartCfg &cfg, const Coef &coef, T *d, T *u,
     TRID_CASE(0, 4, 3)
     TRID_CASE(0, 8, 3)
     TRID_CASE(0, 16, 3)
+    if constexpr (Coef::kArrays) {
+      TRID_CASE(0, 4, 4)
+      TRID_CASE(0, 8, 4)
+      TRID_CASE(0, 16, 4)
+    }
     TRID_CASE(1, 4, 0)
     TRID_CASE(1, 8, 0)
     TRID_CASE(1, 16, 0)
@@ -186,6 +196,7 @@ inline bool cfg_valid(const PartCfg &cfg, int N) {
   const long long thr = cps * spb;
   if (thr < kWave || thr > 1024) return false;
   if (cfg.R >= 16 && thr > 512) return false;  // register budget of R=16
+  if (cfg.vec == 4 && thr > 512) return false;  // launch bound of general staging
   if (cps * cfg.R < N) return false;
   return true;
 }
@@ -220,6 +231,16 @@ inline bool stage_ok(const PartCfg &c, bool dense, long long nsys, int N) {
 // all chunks of a system in one wave, VEC 3 for systems over several waves.
 inline int staged_vec(const PartCfg &c) { return (1 << c.cps_log2) > kWave ? 3 : 2; }
 
+// General wave staging (VEC 4: any N, spans of any alignment) needs ORDER 0,
+// unit stride and 16-byte aligned array bases (gvec), and either one system
+// per wave or more (cps >= 64) or dense arrays (the systems of a wave
+// adjacent).
+inline bool gstage_ok(const PartCfg &c, bool gvec, bool dense) {
+  if (c.kind != 1 || c.order != 0 || c.vec >= 2 || !gvec) return false;
+  if ((1 << (c.cps_log2 + c.spb_log2)) > 512) return false;
+  return (1 << c.cps_log2) >= kWave || dense;
+}
+
 inline std::string cfg_name(const PartCfg &c, size_t tsize) {
   char buf[128];
   if (c.kind == 2) {
@@ -231,14 +252,15 @@ inline std::string cfg_name(const PartCfg &c, size_t tsize) {
   } else {
     snprintf(buf, sizeof buf, "part<%s,o%d,R%d,cps%d,spb%d%s%s>",
              tsize == 8 ? "double" : "float", c.order, c.R, 1 << c.cps_log2,
-             1 << c.spb_log2, c.vec >= 2 ? ",staged" : (c.vec ? ",vec" : ""),
+             1 << c.spb_log2,
+             c.vec == 4 ? ",gstaged" : (c.vec >= 2 ? ",staged" : (c.vec ? ",vec" : "")),
              c.xcd ? ",xcd" : "");
   }
   return buf;
 }
 
 inline std::vector<PartCfg> candidates(int order, int N, int vec, size_t tsize,
-                                bool dense, long long nsys) {
+                                bool dense, long long nsys, bool gvec = false) {
   std::vector<PartCfg> out;
   const int Rs[3] = {4, 8, 16};
   const int thr[3] = {256, 512, 1024};
@@ -264,6 +286,9 @@ inline std::vector<PartCfg> candidates(int order, int N, int vec, size_t tsize,
       add_both(c);
       if (stage_ok(c, dense, nsys, N)) {
         c.vec = staged_vec(c);
+        add_both(c);
+      } else if (gstage_ok(c, gvec, dense)) {
+        c.vec = 4;
         add_both(c);
       }
     }
@@ -334,8 +359,12 @@ inline PartCfg split_cfg(int order, int vec, size_t tsize, long long nsys) {
 }
 
 inline PartCfg heuristic(int order, int N, int vec, size_t tsize) {
-  // preferred chunk lengths: FP64 8 rows, FP32 16 rows (register budget)
-  const int pref[3] = {tsize == 8 ? 8 : 16, tsize == 8 ? 16 : 8, 4};
+  // preferred chunk lengths: FP64 8 rows, FP32 16 rows (register budget);
+  // FP32 along contiguous rows with 16-byte loads: 4 rows, one load per
+  // lane and array, consecutive lanes on consecutive 16 bytes (fastest at
+  // every length measured on MI355X, exact or not)
+  const bool r4 = tsize == 4 && order == 0 && vec;
+  const int pref[3] = {r4 ? 4 : (tsize == 8 ? 8 : 16), tsize == 8 ? 16 : 8, r4 ? 16 : 4};
   for (int R : pref) {
     int v = vec && (R * (int)tsize) % 16 == 0;
     PartCfg c = make_cfg(order, R, N, v, order == 0 ? 256 : 512);

@@ -507,6 +507,92 @@ __device__ __forceinline__ void store_chunk_staged(T *__restrict__ span,
   __builtin_amdgcn_wave_barrier();
 }
 
+// General wave staging (VEC 4): the wave's chunks cover the contiguous
+// element span [start, start + len), len <= 64 R, of an array with a
+// 16-byte aligned base; the span may start and end anywhere.  The 16-byte
+// pieces that cover it (consecutive lanes on consecutive pieces) go to the
+// wave's LDS buffer of 64 R + E elements, so that element start + j sits at
+// buf[sh + j], sh = start mod E.  A piece is the aligned 16-byte block that
+// holds a span element, so it never reaches past the page of that element.
+// Returns sh.
+template <typename T, int R, typename I>
+__device__ __forceinline__ int gspan_load(const T *__restrict__ base, I start, int len,
+                                          T *__restrict__ buf, int lane) {
+  using V = typename Vec16<T>::type;
+  constexpr int E = Vec16<T>::n;
+  const int sh = (int)(start & (I)(E - 1));
+  const V *g = reinterpret_cast<const V *>(at(base, start - (I)sh));
+  const int np = (sh + len + E - 1) / E;
+  V *s = reinterpret_cast<V *>(buf);
+  V t[R / E + 1];
+  // every lane loads R/E + 1 pieces; those past the span repeat its last
+  // piece (unconditional loads schedule far better than predicated ones)
+#pragma unroll
+  for (int v = 0; v <= R / E; v++) {
+    const int pc = v * kWave + lane;
+    t[v] = ld_nt(g + (pc < np ? pc : np - 1));
+  }
+#pragma unroll
+  for (int v = 0; v <= R / E; v++) {
+    const int pc = v * kWave + lane;
+    if (pc < np) s[pc] = t[v];
+  }
+  __builtin_amdgcn_wave_barrier();
+  return sh;
+}
+
+// The lane's `rows` valid elements from buf[i0 ...] (pad after them).
+template <typename T, int R>
+__device__ __forceinline__ void gspan_take(const T *__restrict__ buf, int i0, int rows,
+                                           T (&x)[R], T pad) {
+#pragma unroll
+  for (int r = 0; r < R; r++) x[r] = r < rows ? buf[i0 + r] : pad;
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Writes the span [start, start + len) from buf (element start + j at
+// buf[sh + j], sh = start mod E, filled by the lanes before a wave barrier):
+// pieces wholly inside the span as 16-byte stores, the elements of the two
+// boundary pieces one by one (the rest of those pieces belongs to other
+// waves).  add: the span receives its previous contents (or, with src,
+// those of src) plus buf.
+template <typename T, int R, typename I>
+__device__ __forceinline__ void gspan_store(T *__restrict__ base, I start, int len,
+                                            const T *__restrict__ buf, int lane, bool add,
+                                            const T *src = nullptr) {
+  using V = typename Vec16<T>::type;
+  constexpr int E = Vec16<T>::n;
+  const int sh = (int)(start & (I)(E - 1));
+  V *g = reinterpret_cast<V *>(at(base, start - (I)sh));
+  const T *srcb = src ? src : base;
+  const V *gs = reinterpret_cast<const V *>(at(srcb, start - (I)sh));
+  const int np = (sh + len + E - 1) / E;
+  const V *s = reinterpret_cast<const V *>(buf);
+#pragma unroll
+  for (int v = 0; v <= R / E; v++) {
+    const int pc = v * kWave + lane;
+    if (pc < np) {
+      const int lo = pc * E - sh;  // span index of the piece's first element
+      V w = s[pc];
+      if (lo >= 0 && lo + E <= len) {
+        if (add) w += ld_nt(gs + pc);
+        st_nt(g + pc, w);
+      } else {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const int j = lo + e;
+          if (j >= 0 && j < len) {
+            T val = w[e];
+            if (add) val += ld_nt(at(srcb, start + (I)j));
+            st_nt(at(base, start + (I)j), val);
+          }
+        }
+      }
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
 // Wave-staged load of 64*R contiguous elements with ordinary (cached) loads:
 // a stencil re-reads the span as a neighbour of other lines, so it stays
 // cacheable.  Split in two so that a kernel can have the global loads of
@@ -695,7 +781,10 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
                                            SegOut<T> so = SegOut<T>()) {
   static_assert(!Coef::kHook || VEC == 0, "hooked coefficients load scalars");
   static_assert(MODE == 0 || Coef::kArrays, "segment reduction reads coefficient arrays");
-  constexpr bool STAGED = VEC >= 2;
+  constexpr bool STAGED = VEC == 2 || VEC == 3;
+  constexpr bool GSTAGE = VEC == 4;
+  // per-wave staging buffer (elements)
+  constexpr int SW = kWave * R + (GSTAGE ? Vec16<T>::n : 0);
   const int tid = threadIdx.x;
   const int cps = 1 << cps_log2;
   const int spb = 1 << spb_log2;
@@ -728,6 +817,39 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
   T *stage = nullptr;
   I wave_off[5];
   bool wave_active = false;
+  // general staging: the wave's span of wave_len elements starts at
+  // wave_off[t] in array t; the lane's chunk is at span index g_idx, with
+  // g_rows valid rows
+  int wave_len = 0, g_idx = 0, g_rows = 0;
+  if (GSTAGE) {
+    stage = reinterpret_cast<T *>(smem_raw) + (tid >> 6) * SW;
+    const long long q0 = ((long long)__builtin_amdgcn_readfirstlane((int)(q >> 32)) << 32) |
+                         (unsigned)__builtin_amdgcn_readfirstlane((int)(q & 0xffffffffLL));
+    const int row0w = __builtin_amdgcn_readfirstlane(row0);
+    wave_active = q0 < D.nsys;
+    if (cps <= kWave) {
+      // whole systems, dense: system q at q * N in every array
+      const long long nact = D.nsys - q0 < (kWave >> cps_log2) ? D.nsys - q0 : (kWave >> cps_log2);
+      wave_len = (int)(nact * N);
+      g_idx = (int)(q - q0) * N + row0;
+    } else {
+      // 64 chunks of one system
+      wave_len = N - row0w < kWave * R ? N - row0w : kWave * R;
+      g_idx = row0 - row0w;
+    }
+    g_rows = active ? (N - row0 < R ? N - row0 : R) : 0;
+#pragma unroll
+    for (int t = 0; t < 5; t++) {
+      const I o = off[t] + (I)row0;
+      if constexpr (sizeof(I) == 4) {
+        wave_off[t] = (I)__builtin_amdgcn_readfirstlane((int)o);
+      } else {
+        int lo = __builtin_amdgcn_readfirstlane((int)(o & 0xffffffffLL));
+        int hi = __builtin_amdgcn_readfirstlane((int)(o >> 32));
+        wave_off[t] = (I)(((long long)hi << 32) | (unsigned)lo);
+      }
+    }
+  }
   if (STAGED) {
     // the wave's span starts at the element of its lane 0 (VEC 2: chunk 0
     // of the wave's first system; VEC 3: chunk 64 w of the wave's system,
@@ -770,7 +892,33 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
       c[r] = in ? coef.c : T(0);
     }
   }
-  if (STAGED) {
+  if (GSTAGE) {
+    const int lane = tid & (kWave - 1);
+    if (wave_active) {
+      if constexpr (Coef::kArrays) {
+        int sh = gspan_load<T, R, I>(coef.A, wave_off[0], wave_len, stage, lane);
+        gspan_take<T, R>(stage, sh + g_idx, g_rows, a, T(0));
+        sh = gspan_load<T, R, I>(coef.B, wave_off[1], wave_len, stage, lane);
+        gspan_take<T, R>(stage, sh + g_idx, g_rows, b, T(1));
+        sh = gspan_load<T, R, I>(coef.C, wave_off[2], wave_len, stage, lane);
+        gspan_take<T, R>(stage, sh + g_idx, g_rows, c, T(0));
+      }
+      const int sh = gspan_load<T, R, I>(Dd, wave_off[3], wave_len, stage, lane);
+      gspan_take<T, R>(stage, sh + g_idx, g_rows, d, T(0));
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        a[r] = T(0);
+        b[r] = T(1);
+        c[r] = T(0);
+        d[r] = T(0);
+      }
+    }
+    if (row0 == 0) a[0] = T(0);
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      if (row0 + r == N - 1) c[r] = T(0);
+  } else if (STAGED) {
     // host guarantees: every chunk of an active wave is full and in range
     const int lane = tid & (kWave - 1);
     if (wave_active) {
@@ -936,10 +1084,10 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
   T xf, xl;
   if (ORDER == 0 && cps <= kWave) {
     reduced_wave<T>(f, l, k, cps, tid & (kWave - 1), xf, xl);
-  } else if (ORDER == 0 && VEC == 3) {
+  } else if (ORDER == 0 && (VEC == 3 || VEC == 4)) {
     // segment rows after the wave staging buffers
     reduced_segs<T>(f, l, k, cps, slot,
-                    reinterpret_cast<T *>(smem_raw) + (size_t)blockDim.x * R, xf, xl);
+                    reinterpret_cast<T *>(smem_raw) + (size_t)(blockDim.x >> 6) * SW, xf, xl);
   } else if (cps <= kWave) {
     reduced_lds_w<T>(f, l, k, cps, slot, spb, reinterpret_cast<T *>(smem_raw), xf, xl);
   } else {
@@ -955,7 +1103,22 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
   // receives u + solution
   T *__restrict__ Dst = Out ? Out : Dd;
   T *__restrict__ Udst = Out ? Out : U;
-  if (STAGED) {
+  if (GSTAGE) {
+    const int lane = tid & (kWave - 1);
+    if (wave_active) {
+      const int t = INC ? 4 : 3;
+      const int sh = (int)(wave_off[t] & (I)(Vec16<T>::n - 1));
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (r < g_rows) stage[sh + g_idx + r] = y[r];
+      __builtin_amdgcn_wave_barrier();
+      if (INC)
+        gspan_store<T, R, I>(Udst, wave_off[4], wave_len, stage, lane, true,
+                             static_cast<const T *>(U));
+      else
+        gspan_store<T, R, I>(Dst, wave_off[3], wave_len, stage, lane, false);
+    }
+  } else if (STAGED) {
     const int lane = tid & (kWave - 1);
     if (wave_active) {
       if (INC)

@@ -298,14 +298,15 @@ tridStatus_t run_cfg(const PartCfg &cfg, const T *a, const T *b, const T *c,
 }
 
 template <typename T, int INC>
-PartCfg autotune(int order, int vec, int idx32, bool dense, const T *a,
+PartCfg autotune(int order, int vec, bool gvec, int idx32, bool dense, const T *a,
                  const T *b, const T *c, T *d, T *u, const SysDesc &D,
                  long long d_extent, hipStream_t st) {
   std::vector<PartCfg> cands =
-      candidates(order, D.N, vec, sizeof(T), dense, D.nsys);
+      candidates(order, D.N, vec, sizeof(T), dense, D.nsys, gvec);
   for (auto &cf : cands) cf.idx32 = idx32;
   PartCfg best = heuristic(order, D.N, vec, sizeof(T));
   if (stage_ok(best, dense, D.nsys, D.N)) best.vec = staged_vec(best);
+  else if (gstage_ok(best, gvec, dense)) best.vec = 4;
   best.idx32 = idx32;
   if (cands.size() < 2) return best;
   // the candidates run on the caller's own arrays (memory-channel effects of
@@ -399,6 +400,13 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
       if (D.ostr[t][k] % E != 0) vec = 0;
   }
 
+  // general wave staging needs only unit stride and 16-byte aligned bases
+  bool gvec = order == 0;
+  for (int t = 0; t < 5 && gvec; t++) {
+    if (t == 4 && !INC) break;
+    if (!aligned16(ptrs[t])) gvec = false;
+  }
+
   // dense: every array is unpadded (system q of a solvedim-0 batch at q*N)
   bool dense = true;
   for (int t = 0; t < 5; t++) {
@@ -433,10 +441,11 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
     if (!cfg_valid(cfg, D.N))
       return fail(TRID_STATUS_INVALID_VALUE, "forced configuration does not fit this shape");
     if (fc.stage && stage_ok(cfg, dense, D.nsys, D.N)) cfg.vec = staged_vec(cfg);
+    else if (fc.stage && gstage_ok(cfg, gvec, dense)) cfg.vec = 4;
     cfg.xcd = fc.kind == 3;
   } else if (autotune_mode() == 1) {
     auto key = std::make_tuple((int)sizeof(T), INC, order, D.N, D.nsys,
-                               vec * 2 + (int)dense, D.sstr[3],
+                               vec * 4 + (int)gvec * 2 + (int)dense, D.sstr[3],
                                D.nother > 0 ? D.ostr[3][0] : 0LL);
     bool have = false;
     {
@@ -449,14 +458,27 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
     }
     if (!have) {
       long long ext = extent(ndim, dims, INC ? pads[4] : pads[3]);
-      cfg = autotune<T, INC>(order, vec, idx32, dense, a, b, c, d, u, D, ext,
+      cfg = autotune<T, INC>(order, vec, gvec, idx32, dense, a, b, c, d, u, D, ext,
                              st);
       std::lock_guard<std::mutex> lk(g_tune_mu);
       g_tuned[key] = cfg;
     }
   } else {
     cfg = heuristic(order, D.N, vec, sizeof(T));
-    if (stage_ok(cfg, dense, D.nsys, D.N)) cfg.vec = staged_vec(cfg);
+    // staging pays when a chunk is more than one 16-byte vector
+    if (cfg.R * sizeof(T) > 16) {
+      if (stage_ok(cfg, dense, D.nsys, D.N)) {
+        cfg.vec = staged_vec(cfg);
+      } else {
+        // general staging takes up to 512 threads: longer FP64 chunks if
+        // needed (FP32 is faster unstaged than in 16-row chunks)
+        PartCfg c16 = make_cfg(order, 16, D.N, cfg.vec, 256);
+        if (sizeof(T) == 8 && !gstage_ok(cfg, gvec, dense) && cfg_valid(c16, D.N) &&
+            gstage_ok(c16, gvec, dense))
+          cfg = c16;
+        if (gstage_ok(cfg, gvec, dense)) cfg.vec = 4;
+      }
+    }
   }
   // systems longer than one work-group holds: split-line solve
   if (cfg.kind == 4 && fc.kind != 4) cfg = split_cfg(order, vec, sizeof(T), D.nsys);
