@@ -5,10 +5,10 @@
 // trid_part.hpp (chunks of R = 8 rows per lane, reduced system by PCR), with the number of chunks per system (cps,
 // 1..512) chosen per length bin.  A work item is one work-group of 512
 // threads processing 512/cps systems of one bin.
-// Loads are staged per system group through LDS: the cps lanes of a group
-// read the system's contiguous span with consecutive lanes on consecutive
-// elements, then each lane takes its chunk from LDS (the same transpose as
-// the solvedim-0 path, for spans at arbitrary offsets).
+// Loads are staged per span group through LDS: the lanes of a system (or of
+// one wave's 64 chunks of a longer system) read its contiguous span as
+// 16-byte pieces, then each lane takes its chunk from LDS (the transpose of
+// the solvedim-0 path, for spans at arbitrary offsets and alignments).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,51 +54,92 @@ struct RagArgs {
   unsigned *counter;  // [0] next item, [1] finished work-groups
 };
 
-// Barrier over the lanes of one system group: a wave barrier when the
-// group fits in one wave (cps <= 64; LDS operations of a wave execute in
-// order), the work-group barrier otherwise.
-__device__ __forceinline__ void rag_sync(int cps) {
-  if (cps > kWave)
-    __syncthreads();
-  else
-    __builtin_amdgcn_wave_barrier();
+// Span-group staging.  A span group is the G lanes that hold one contiguous
+// row range of one system: the whole system when cps <= 64 (G = cps), else
+// the 64 chunks of one wave (G = 64), so a group never leaves its wave and
+// no work-group barrier is needed.  The group reads its span [p, p + len)
+// of an array as the 16-byte aligned pieces that cover it (piece j at group
+// lane j mod G; lanes past the span repeat its last piece, which keeps the
+// loads unpredicated), into the group's LDS region of G*R + E elements, so
+// that element p + i sits at reg[sh + i], sh = (address of p / sizeof(T))
+// mod E.  A piece is the aligned block that holds a span element, so no
+// load reaches past that element's page.  Returns sh.
+template <typename T, int R>
+__device__ __forceinline__ int rag_load(const T *__restrict__ p, int len, T *__restrict__ reg,
+                                        int kg, int G) {
+  using V = typename Vec16<T>::type;
+  constexpr int E = Vec16<T>::n;
+  const int sh = (int)((reinterpret_cast<uintptr_t>(p) / sizeof(T)) & (E - 1));
+  const V *g = reinterpret_cast<const V *>(p - sh);
+  const int np = len > 0 ? (sh + len + E - 1) / E : 0;
+  const int last = np > 0 ? np - 1 : 0;
+  V t[R / E + 1];
+#pragma unroll
+  for (int j = 0; j <= R / E; j++) {
+    const int pc = j * G + kg;
+    t[j] = ld_nt(g + (pc < np ? pc : last));
+  }
+  V *s = reinterpret_cast<V *>(reg);
+#pragma unroll
+  for (int j = 0; j <= R / E; j++) {
+    const int pc = j * G + kg;
+    if (pc < np) s[pc] = t[j];
+  }
+  __builtin_amdgcn_wave_barrier();
+  return sh;
 }
 
-// Group-staged access: the cps lanes of a system group read the system's
-// span [off, off+L) with consecutive lanes on consecutive elements (element
-// r*cps + k in the lane's slot r), then every lane takes its own chunk
-// (elements k*R .. k*R+R-1) from the group's LDS region.
+// The lane's rows i0 .. i0+R-1 of the span (pad past len) from reg.
 template <typename T, int R>
-__device__ __forceinline__ void rag_issue(const T *__restrict__ X, long long off, int L, int k,
-                                          int cps, T (&g)[R]) {
+__device__ __forceinline__ void rag_take(const T *reg, int sh, int i0, int len, T (&x)[R],
+                                         T pad) {
 #pragma unroll
-  for (int r = 0; r < R; r++) {
-    const int e = r * cps + k;
-    g[r] = e < L ? X[off + e] : T(0);
-  }
+  for (int r = 0; r < R; r++) x[r] = i0 + r < len ? reg[sh + i0 + r] : pad;
+  __builtin_amdgcn_wave_barrier();
 }
 
+// Writes the span [p, p + len) from reg (filled by the lanes, then a wave
+// barrier): pieces wholly inside the span as 16-byte stores, the elements of
+// the boundary pieces one by one (the rest of them belongs to other
+// systems).  add: p receives its previous contents plus reg.
 template <typename T, int R>
-__device__ __forceinline__ void rag_put(const T (&g)[R], int L, int k, int cps, T *st) {
+__device__ __forceinline__ void rag_store(T *__restrict__ p, int len, const T *__restrict__ reg,
+                                          int kg, int G, bool add) {
+  using V = typename Vec16<T>::type;
+  constexpr int E = Vec16<T>::n;
+  const int sh = (int)((reinterpret_cast<uintptr_t>(p) / sizeof(T)) & (E - 1));
+  V *g = reinterpret_cast<V *>(p - sh);
+  const int np = len > 0 ? (sh + len + E - 1) / E : 0;
+  const V *s = reinterpret_cast<const V *>(reg);
 #pragma unroll
-  for (int r = 0; r < R; r++) {
-    const int e = r * cps + k;
-    if (e < L) st[e] = g[r];
-  }
-}
-
-template <typename T, int R>
-__device__ __forceinline__ void rag_take(const T *st, int L, int k, T (&x)[R], T pad) {
+  for (int j = 0; j <= R / E; j++) {
+    const int pc = j * G + kg;
+    if (pc < np) {
+      const int lo = pc * E - sh;
+      V w = s[pc];
+      if (lo >= 0 && lo + E <= len) {
+        if (add) w += ld_nt(g + pc);
+        st_nt(g + pc, w);
+      } else {
 #pragma unroll
-  for (int r = 0; r < R; r++) {
-    const int e = k * R + r;
-    x[r] = e < L ? st[e] : pad;
+        for (int e = 0; e < E; e++) {
+          const int i = lo + e;
+          if (i >= 0 && i < len) {
+            T v = w[e];
+            if (add) v += ld_nt(p + i);
+            st_nt(p + i, v);
+          }
+        }
+      }
+    }
   }
+  __builtin_amdgcn_wave_barrier();
 }
 
 template <typename T>
 __device__ __forceinline__ void rag_item(const RagArgs<T> &g, const RagItem &it, T *smem) {
   constexpr int R = rag_rows<T>();
+  constexpr int E = Vec16<T>::n;
   const int tid = threadIdx.x;
   const int cps = 1 << it.cps_log2;
   const int k = tid & (cps - 1);
@@ -111,27 +152,26 @@ __device__ __forceinline__ void rag_item(const RagArgs<T> &g, const RagItem &it,
     off = g.offsets[id];
     L = (int)(g.offsets[id + 1] - off);
   }
-  // two staging buffers of kRagThreads*R elements; the group's region
-  T *st0 = smem + slot * (cps * R);
-  T *st1 = st0 + kRagThreads * R;
-  // all four arrays' global loads in flight before the first transpose
+  // the lane's span group: G lanes, rows row_base .. of the system
+  const int G = cps < kWave ? cps : kWave;
+  const int kg = k & (G - 1);
+  const int row_base = (k - kg) * R;
+  const int len = L - row_base < 0 ? 0 : (L - row_base < G * R ? L - row_base : G * R);
+  T *reg = smem + (tid / G) * (G * R + E);
+  // a group with nothing to load (inactive slot, or a wave past its
+  // system's end) points its loads at element 0, which every array has (the
+  // plan holds only systems of at least one row)
+  const long long p0 = len > 0 ? off + row_base : 0;
+  const int i0 = kg * R;  // the lane's first row within the span
   T a[R], b[R], c[R], d[R];
-  rag_issue<T, R>(g.a, off, L, k, cps, a);
-  rag_issue<T, R>(g.b, off, L, k, cps, b);
-  rag_issue<T, R>(g.c, off, L, k, cps, c);
-  rag_issue<T, R>(g.d, off, L, k, cps, d);
-  rag_put<T, R>(a, L, k, cps, st0);
-  rag_put<T, R>(b, L, k, cps, st1);
-  rag_sync(cps);
-  rag_take<T, R>(st0, L, k, a, T(0));
-  rag_take<T, R>(st1, L, k, b, T(1));
-  rag_sync(cps);
-  rag_put<T, R>(c, L, k, cps, st0);
-  rag_put<T, R>(d, L, k, cps, st1);
-  rag_sync(cps);
-  rag_take<T, R>(st0, L, k, c, T(0));
-  rag_take<T, R>(st1, L, k, d, T(0));
-  rag_sync(cps);
+  int sh = rag_load<T, R>(g.a + p0, len, reg, kg, G);
+  rag_take<T, R>(reg, sh, i0, len, a, T(0));
+  sh = rag_load<T, R>(g.b + p0, len, reg, kg, G);
+  rag_take<T, R>(reg, sh, i0, len, b, T(1));
+  sh = rag_load<T, R>(g.c + p0, len, reg, kg, G);
+  rag_take<T, R>(reg, sh, i0, len, c, T(0));
+  sh = rag_load<T, R>(g.d + p0, len, reg, kg, G);
+  rag_take<T, R>(reg, sh, i0, len, d, T(0));
   const int row0 = k * R;
   if (row0 == 0) a[0] = T(0);
 #pragma unroll
@@ -145,28 +185,20 @@ __device__ __forceinline__ void rag_item(const RagArgs<T> &g, const RagItem &it,
     reduced_wave<T>(f, l, k, cps, tid & (kWave - 1), xf, xl);
   } else {
     // the PCR exchange buffers (6 values per thread) reuse the staging
-    // region; the loads above are complete (barrier)
+    // region once every wave is done with it
+    __syncthreads();
     reduced_lds<T>(f, l, k, cps, slot, kRagThreads >> it.cps_log2, smem, xf, xl);
     __syncthreads();
   }
   T y[R];
   chunk_expand<T, R>(a, c, d, xf, xl, y);
+  T *dst = (g.inc ? g.u : g.d) + p0;
+  const int shd = (int)((reinterpret_cast<uintptr_t>(dst) / sizeof(T)) & (E - 1));
 #pragma unroll
-  for (int r = 0; r < R; r++) {
-    const int e = row0 + r;
-    if (e < L) st0[e] = y[r];
-  }
-  rag_sync(cps);
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    const int e = r * cps + k;
-    if (e < L) {
-      if (g.inc)
-        g.u[off + e] += st0[e];
-      else
-        g.d[off + e] = st0[e];
-    }
-  }
+  for (int r = 0; r < R; r++)
+    if (i0 + r < len) reg[shd + i0 + r] = y[r];
+  __builtin_amdgcn_wave_barrier();
+  rag_store<T, R>(dst, len, reg, kg, G, g.inc != 0);
 }
 
 // Persistent work-groups take items from an atomic counter; the next
@@ -378,7 +410,9 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
   p->nlong = (int)longs.size();
   p->long_ws = lws;
   const size_t ts = p->dbl ? 8 : 4;
-  p->lds = 2 * (size_t)kRagThreads * R * ts;  // two staging buffers
+  // span-group regions (at most 512 groups of G*R + E elements, G >= 1);
+  // the LDS reduced solve reuses them (6 values per thread)
+  p->lds = (size_t)kRagThreads * (R + 16 / ts) * ts;
   auto up = [&](void **dst, const void *src, size_t bytes) -> tridStatus_t {
     if (bytes == 0) return TRID_STATUS_SUCCESS;
     tridStatus_t r = check_hip(hipMalloc(dst, bytes), "hipMalloc");
