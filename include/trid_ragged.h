@@ -9,12 +9,16 @@
  * solved exactly as trid_scalarD would solve it: a of its first row and c
  * of its last row are never read; non-Inc overwrites d, Inc adds into u.
  *
+ * The arrays must hold at least offsets[nsys] elements: the library sees
+ * only the offsets and accesses every element they name.
+ *
  * tridRaggedPlanCreate reads the offsets once (device array of nsys+1
  * int64 values, non-decreasing), bins the systems by length into chunked
- * schedules (chunks of 8 rows; one lane per system up to 8 rows, 2..64
- * lanes per system with an in-wave reduced solve, 128..512 lanes per system
- * across the waves of a work-group through LDS) and builds a work list
- * ordered from the most to the least expensive item.  Each solve is one
+ * schedules (a power-of-two number of chunks of 5..8 rows; one lane per
+ * system up to 8 rows, 2..64 lanes per system with an in-wave reduced
+ * solve, 128..512 lanes per system across the waves of a work-group through
+ * LDS) and builds a work list ordered from the most to the least expensive
+ * item.  Each solve is one
  * launch of a persistent kernel whose work-groups pull items from an atomic
  * counter (reset by the kernel itself).  Systems longer than 4096 rows are
  * solved by a second, one-thread-per-system launch.

@@ -76,6 +76,26 @@ def test_ragged_every_bin(dtype, inc):
     check_close(exp_u if inc else exp_d, got, dtype, f"ragged [{plan.describe()}]")
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_ragged_every_chunk_length(dtype):
+    """Every bin (chunks per system 1..512, chunk length 5..8 rows) at both
+    of its edges: cps*(R-1)+1 and cps*R rows."""
+    lengths = []
+    for cl in range(10):
+        cps = 1 << cl
+        for R in range(5, 9):
+            lengths += [cps * (R - 1) + 1, cps * R]
+    lengths = np.array([L for L in lengths if L <= 4096] * 2, dtype=np.int64)
+    np.random.default_rng(9).shuffle(lengths)
+    off, arrs = ragged_inputs(lengths, dtype, seed=3)
+    (got,), plan = run(off, arrs, dtype)
+    exp = arrs[3].copy()
+    assert oracle.ragged(arrs[0], arrs[1], arrs[2], exp, None, off) == 0
+    check_close(exp, got, dtype, f"ragged [{plan.describe()}]")
+    for R in range(5, 9):
+        assert f"R{R}/" in plan.describe()
+
+
 def test_ragged_repeated_solves_one_plan():
     lengths = np.array([5, 300, 1200, 17, 4000, 64, 9000] * 40, dtype=np.int64)
     off, arrs = ragged_inputs(lengths, np.float64, seed=2)

@@ -27,19 +27,18 @@ namespace tridhip {
 constexpr int kRagThreads = 512;
 constexpr int kRagMaxLen = 4096;  // longest system of the persistent kernel
 
-// Chunk length of the persistent kernel: every bin uses it, only the chunks
-// per system (1..512) change, so one kernel serves all bins at the register
-// cost of one chunk length (512 chunks of 8 rows cover 4096 rows).
-template <typename T>
-constexpr int rag_rows() {
-  return 8;
-}
+// Chunk lengths of the persistent kernel.  A bin is (cps, R): cps a power of
+// two (1..512) and R the shortest of 5..8 rows with cps * R >= L, so that at
+// least 4/5 of a system's lanes hold rows (a power-of-two chunk count with
+// one fixed R leaves a quarter idle on average).  The kernel's registers are
+// those of the longest, 8 rows (512 chunks of 8 rows cover 4096 rows).
+constexpr int kRagRmin = 5, kRagRmax = 8;
 
 struct RagItem {
   int cps_log2;  // chunks per system
   int first;     // first entry of the item in the sorted system list
   int count;     // systems in this item (<= kRagThreads / cps)
-  int pad;
+  int rows;      // chunk length R of the item's bin (5..8)
 };
 
 template <typename T>
@@ -60,10 +59,13 @@ struct RagArgs {
 // no work-group barrier is needed.  The group reads its span [p, p + len)
 // of an array as the 16-byte aligned pieces that cover it (piece j at group
 // lane j mod G; lanes past the span repeat its last piece, which keeps the
-// loads unpredicated), into the group's LDS region of G*R + E elements, so
+// loads unpredicated), into the group's LDS region of G*R + 2E elements, so
 // that element p + i sits at reg[sh + i], sh = (address of p / sizeof(T))
-// mod E.  A piece is the aligned block that holds a span element, so no
-// load reaches past that element's page.  Returns sh.
+// mod E.  A span of G*R elements needs at most G*R/E + 1 pieces (at most
+// G*R + 2E - 2 elements), hence ceil(R/E) + 1 rounds of G lanes and the 2E
+// spare elements when R is not a multiple of E.  A piece is the aligned
+// block that holds a span element, so no load reaches past that element's
+// page.  Returns sh.
 template <typename T, int R>
 __device__ __forceinline__ int rag_load(const T *__restrict__ p, int len, T *__restrict__ reg,
                                         int kg, int G) {
@@ -73,15 +75,16 @@ __device__ __forceinline__ int rag_load(const T *__restrict__ p, int len, T *__r
   const V *g = reinterpret_cast<const V *>(p - sh);
   const int np = len > 0 ? (sh + len + E - 1) / E : 0;
   const int last = np > 0 ? np - 1 : 0;
-  V t[R / E + 1];
+  constexpr int NR = (R + E - 1) / E + 1;
+  V t[NR];
 #pragma unroll
-  for (int j = 0; j <= R / E; j++) {
+  for (int j = 0; j < NR; j++) {
     const int pc = j * G + kg;
     t[j] = ld_nt(g + (pc < np ? pc : last));
   }
   V *s = reinterpret_cast<V *>(reg);
 #pragma unroll
-  for (int j = 0; j <= R / E; j++) {
+  for (int j = 0; j < NR; j++) {
     const int pc = j * G + kg;
     if (pc < np) s[pc] = t[j];
   }
@@ -111,8 +114,9 @@ __device__ __forceinline__ void rag_store(T *__restrict__ p, int len, const T *_
   V *g = reinterpret_cast<V *>(p - sh);
   const int np = len > 0 ? (sh + len + E - 1) / E : 0;
   const V *s = reinterpret_cast<const V *>(reg);
+  constexpr int NR = (R + E - 1) / E + 1;
 #pragma unroll
-  for (int j = 0; j <= R / E; j++) {
+  for (int j = 0; j < NR; j++) {
     const int pc = j * G + kg;
     if (pc < np) {
       const int lo = pc * E - sh;
@@ -136,9 +140,8 @@ __device__ __forceinline__ void rag_store(T *__restrict__ p, int len, const T *_
   __builtin_amdgcn_wave_barrier();
 }
 
-template <typename T>
+template <typename T, int R>
 __device__ __forceinline__ void rag_item(const RagArgs<T> &g, const RagItem &it, T *smem) {
-  constexpr int R = rag_rows<T>();
   constexpr int E = Vec16<T>::n;
   const int tid = threadIdx.x;
   const int cps = 1 << it.cps_log2;
@@ -157,7 +160,7 @@ __device__ __forceinline__ void rag_item(const RagArgs<T> &g, const RagItem &it,
   const int kg = k & (G - 1);
   const int row_base = (k - kg) * R;
   const int len = L - row_base < 0 ? 0 : (L - row_base < G * R ? L - row_base : G * R);
-  T *reg = smem + (tid / G) * (G * R + E);
+  T *reg = smem + (tid / G) * (G * R + 2 * E);
   // a group with nothing to load (inactive slot, or a wave past its
   // system's end) points its loads at element 0, which every array has (the
   // plan holds only systems of at least one row)
@@ -214,7 +217,13 @@ __global__ void __launch_bounds__(kRagThreads) k_ragged(RagArgs<T> g) {
   while (i < g.nitems) {
     __syncthreads();  // every lane has read s_next
     if (threadIdx.x == 0) s_next = (int)atomicAdd(g.counter, 1u);
-    rag_item<T>(g, g.items[i], smem);
+    const RagItem it = g.items[i];
+    switch (it.rows) {
+      case 5: rag_item<T, 5>(g, it, smem); break;
+      case 6: rag_item<T, 6>(g, it, smem); break;
+      case 7: rag_item<T, 7>(g, it, smem); break;
+      default: rag_item<T, 8>(g, it, smem); break;
+    }
     __syncthreads();  // s_next written; the next item reuses the staging region
     i = s_next;
   }
@@ -360,15 +369,16 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
       free_plan(p);
       return fail(TRID_STATUS_INVALID_VALUE, "offsets must be non-decreasing");
     }
-  // bins by chunks per system (cps = 2^cl >= L / R), one chunk length R
-  const int R = p->dbl ? rag_rows<double>() : rag_rows<float>();
-  auto bin_of = [&](long long L) {
-    const long long ch = (L + R - 1) / R;
+  // bins (cps = 2^cl >= L / 8, R = the shortest of 5..8 rows with cps*R >= L)
+  auto bin_of = [&](long long L, int &R) {
+    const long long ch = (L + kRagRmax - 1) / kRagRmax;
     int cl = 0;
     while ((1LL << cl) < ch) cl++;
+    R = (int)std::max<long long>(kRagRmin, (L + (1LL << cl) - 1) >> cl);
     return cl;
   };
-  std::vector<std::vector<int>> bins(16);
+  constexpr int nR = kRagRmax - kRagRmin + 1;
+  std::vector<std::vector<int>> bins(16 * nR);
   std::vector<int> longs;
   std::vector<long long> long_off;
   long long lws = 0;
@@ -381,38 +391,41 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
       lws += 2 * L;
       continue;
     }
-    bins[bin_of(L)].push_back((int)k);
+    int R = 0;
+    const int cl = bin_of(L, R);
+    bins[cl * nR + (R - kRagRmin)].push_back((int)k);
   }
-  // items, most expensive first (more chunks per system first)
+  // items, most expensive first (more chunks per system, then longer chunks)
   std::vector<int> sys;
   std::vector<RagItem> items;
   std::string desc;
-  for (int cl = 15; cl >= 0; cl--) {
-    std::vector<int> &v = bins[cl];
-    if (v.empty()) continue;
-    const int per = std::max(1, kRagThreads >> cl);
-    for (size_t i = 0; i < v.size(); i += per) {
-      RagItem it;
-      it.cps_log2 = cl;
-      it.first = (int)sys.size() + (int)i;
-      it.count = (int)std::min<size_t>(per, v.size() - i);
-      it.pad = 0;
-      items.push_back(it);
+  for (int cl = 15; cl >= 0; cl--)
+    for (int R = kRagRmax; R >= kRagRmin; R--) {
+      std::vector<int> &v = bins[cl * nR + (R - kRagRmin)];
+      if (v.empty()) continue;
+      const int per = std::max(1, kRagThreads >> cl);
+      for (size_t i = 0; i < v.size(); i += per) {
+        RagItem it;
+        it.cps_log2 = cl;
+        it.first = (int)sys.size() + (int)i;
+        it.count = (int)std::min<size_t>(per, v.size() - i);
+        it.rows = R;
+        items.push_back(it);
+      }
+      sys.insert(sys.end(), v.begin(), v.end());
+      char buf[96];
+      snprintf(buf, sizeof buf, "R%d/cps%d:%zu ", R, 1 << cl, v.size());
+      desc += buf;
     }
-    sys.insert(sys.end(), v.begin(), v.end());
-    char buf[96];
-    snprintf(buf, sizeof buf, "R%d/cps%d:%zu ", R, 1 << cl, v.size());
-    desc += buf;
-  }
   if (!longs.empty()) desc += "long(>4096):" + std::to_string(longs.size());
   p->text = desc;
   p->nitems = (int)items.size();
   p->nlong = (int)longs.size();
   p->long_ws = lws;
   const size_t ts = p->dbl ? 8 : 4;
-  // span-group regions (at most 512 groups of G*R + E elements, G >= 1);
+  // span-group regions (at most 512 groups of G*R + 2E elements, G >= 1);
   // the LDS reduced solve reuses them (6 values per thread)
-  p->lds = (size_t)kRagThreads * (R + 16 / ts) * ts;
+  p->lds = (size_t)kRagThreads * (kRagRmax + 2 * (16 / ts)) * ts;
   auto up = [&](void **dst, const void *src, size_t bytes) -> tridStatus_t {
     if (bytes == 0) return TRID_STATUS_SUCCESS;
     tridStatus_t r = check_hip(hipMalloc(dst, bytes), "hipMalloc");
