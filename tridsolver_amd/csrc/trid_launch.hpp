@@ -76,12 +76,12 @@ hipError_t launch_part_t(const PartCfg &cfg, const Coef &coef, T *d, T *u,
   // ORDER 1 with cps <= 64, reduced_lds_w (6 * cps * (spb + 1))
   if (ORDER == 1 || cps > kWave) lds = 2 * 3 * (size_t)cps * (spb + 1) * sizeof(T);
   if (VEC == 2) lds = (size_t)threads * R * sizeof(T);  // wave staging buffers
-  // staging buffers, then 8 values per wave of segment rows (reduced_segs)
-  if (VEC == 3) lds = (size_t)threads * R * sizeof(T) + 8 * (size_t)(threads / kWave) * sizeof(T);
+  // staging buffers, then 12 values per wave for the reduced solve
+  if (VEC == 3) lds = (size_t)threads * R * sizeof(T) + 12 * (size_t)(threads / kWave) * sizeof(T);
   // general staging: per wave 64 R + E elements (E = 16 / sizeof(T)), then
-  // the segment rows of systems over several waves
+  // 12 values per wave for the reduced solve of systems over several waves
   if (VEC == 4)
-    lds = (size_t)(threads / kWave) * (kWave * R + 16 / sizeof(T) + (cps > kWave ? 8 : 0)) *
+    lds = (size_t)(threads / kWave) * (kWave * R + 16 / sizeof(T) + (cps > kWave ? 12 : 0)) *
           sizeof(T);
   if (cfg.idx32)
     hipLaunchKernelGGL((k_part<T, INC, ORDER, R, VEC, uint32_t, Coef>),

@@ -291,79 +291,133 @@ __device__ __forceinline__ void reduced_lds_w(const Row<T> &f, const Row<T> &l, 
   xl = sm[n + me];
 }
 
-// In-wave PCR of an n-row tridiagonal system (n <= 64) held one row per lane
-// in lanes 0..n-1 (rows of other lanes are ignored); returns lane j's x_j.
+// A row of the last-unknown system with three right-hand sides: d, and the
+// unit couplings p / q to the unknowns just before and just after a wave.
 template <typename T>
-__device__ __forceinline__ T pcr_lanes(Row<T> r, int n, int lane) {
-  for (int s = 1; s < n; s <<= 1) {
-    Row<T> m = row_from_lane(r, lane - s, lane >= s && lane < n);
-    Row<T> p = row_from_lane(r, lane + s, lane + s < n);
-    r = pcr_update(r, m, p);
-  }
-  return r.d;
+struct Row3 {
+  T a, c, d, p, q;
+};
+
+template <typename T>
+__device__ __forceinline__ Row3<T> row3_from_lane(const Row3<T> &v, int src, bool valid) {
+  const int s = valid ? src : 0;
+  Row3<T> o;
+  o.a = lane_read(v.a, s);
+  o.c = lane_read(v.c, s);
+  o.d = lane_read(v.d, s);
+  o.p = lane_read(v.p, s);
+  o.q = lane_read(v.q, s);
+  if (!valid) o = Row3<T>{T(0), T(0), T(0), T(0), T(0)};
+  return o;
+}
+
+template <typename T>
+__device__ __forceinline__ Row3<T> pcr_update3(const Row3<T> &r, const Row3<T> &m,
+                                               const Row3<T> &n) {
+  const T inv = rcp(T(1) - r.a * m.c - r.c * n.a);
+  Row3<T> o;
+  o.a = -r.a * m.a * inv;
+  o.c = -r.c * n.c * inv;
+  o.d = (r.d - r.a * m.d - r.c * n.d) * inv;
+  o.p = (r.p - r.a * m.p - r.c * n.p) * inv;
+  o.q = (r.q - r.a * m.q - r.c * n.q) * inv;
+  return o;
 }
 
 // Reduced solve for systems whose cps > 64 chunks sit in consecutive lanes
-// of W = cps / 64 whole waves (ORDER 0 and the ragged kernel), in two
-// levels with two barriers (against one per PCR level in reduced_lds):
-//   1. each wave merges its 64 chunks' rows in a tree (seg_merge) into the
-//      two rows of its wave segment, which go to LDS;
-//   2. the first wave of every system solves the system's 2W segment rows
-//      (unknowns: first and last of every wave segment) by in-wave PCR;
-//   3. every wave moves the couplings to its neighbouring segments' solved
-//      unknowns into its first and last chunk rows and solves its 64 chunks
-//      with reduced_wave.
-// LDS: 8 * W values per system slot at sm + slot * 8W.  All threads call it.
+// of W = cps / 64 whole waves (ORDER 0 and the ragged kernel).  The chunks'
+// last unknowns l_k form one tridiagonal system (eliminate_first; the last
+// chunk of a wave takes the next wave's first chunk row through LDS), of
+// which every wave holds 64 consecutive rows.  Three barriers:
+//   1. every wave solves its 64 rows by in-wave PCR with three right-hand
+//      sides, so that l_k = D_k - P_k * X_p - Q_k * X_n in terms of the
+//      unknowns just outside the wave (X_p = l of the previous wave's last
+//      chunk, X_n = l of the next wave's first chunk);
+//   2. one lane per system solves for every wave's first and last l: a
+//      block-tridiagonal system of 2x2 blocks whose forward sweep is scalar
+//      (the coupling blocks have one nonzero column);
+//   3. every lane evaluates its l_k and the chunk's first unknown.
+// LDS: 12 W values per system slot at sm + slot * 12W.  All threads call it.
 template <typename T>
-__device__ __forceinline__ void reduced_segs(const Row<T> &f, const Row<T> &l, int k, int cps,
-                                             int slot, T *sm, T &xf, T &xl) {
+__device__ __forceinline__ void reduced_multi(const Row<T> &f, const Row<T> &l, int k, int cps,
+                                              int slot, T *sm, T &xf, T &xl) {
   const int lane = threadIdx.x & (kWave - 1);
   const int W = cps >> 6;
   const int w = k >> 6;
-  Row<T> F = f, L = l;
-  for (int s = 1; s < kWave; s <<= 1) {
-    const int src = (lane + s) & (kWave - 1);
-    Row<T> fb, lb;
-    fb.a = lane_read(F.a, src);
-    fb.c = lane_read(F.c, src);
-    fb.d = lane_read(F.d, src);
-    lb.a = lane_read(L.a, src);
-    lb.c = lane_read(L.c, src);
-    lb.d = lane_read(L.d, src);
-    if ((lane & (2 * s - 1)) == 0) seg_merge<T>(F, L, fb, lb);
-  }
-  T *rows = sm + slot * (8 * W);  // 2W rows of (a, c, d), then 2W unknowns
-  T *xs = rows + 6 * W;
+  T *fs = sm + slot * (12 * W);  // [W][3]: first row of each wave's chunk 0
+  T *bs = fs + 3 * W;            // [W][6]: D, P, Q of rows 0 and 63; then G, H
+  T *zs = bs + 6 * W;            // [W][2]: first and last l of every wave
   if (lane == 0) {
-    rows[6 * w] = F.a;
-    rows[6 * w + 1] = F.c;
-    rows[6 * w + 2] = F.d;
-    rows[6 * w + 3] = L.a;
-    rows[6 * w + 4] = L.c;
-    rows[6 * w + 5] = L.d;
+    fs[3 * w] = f.a;
+    fs[3 * w + 1] = f.c;
+    fs[3 * w + 2] = f.d;
   }
   __syncthreads();
-  if (w == 0) {
-    const int n = 2 * W;
-    Row<T> r;
-    const int j = lane < n ? lane : 0;
-    r.a = rows[3 * j];
-    r.c = rows[3 * j + 1];
-    r.d = rows[3 * j + 2];
-    const T x = pcr_lanes<T>(r, n, lane);
-    if (lane < n) xs[lane] = x;
-  }
-  __syncthreads();
-  Row<T> f2 = f, l2 = l;
-  if (lane == 0 && w > 0) {
-    f2.d -= f2.a * xs[2 * w - 1];
-    f2.a = T(0);
-  }
+  Row<T> fn = row_from_lane(f, lane + 1, lane + 1 < kWave);
   if (lane == kWave - 1 && w + 1 < W) {
-    l2.d -= l2.c * xs[2 * w + 2];
-    l2.c = T(0);
+    fn.a = fs[3 * w + 3];
+    fn.c = fs[3 * w + 4];
+    fn.d = fs[3 * w + 5];
   }
-  reduced_wave<T>(f2, l2, lane, kWave, lane, xf, xl);
+  const Row<T> r = eliminate_first(f, l, fn);
+  Row3<T> q;
+  q.a = lane == 0 ? T(0) : r.a;
+  q.c = lane == kWave - 1 ? T(0) : r.c;
+  q.d = r.d;
+  q.p = lane == 0 ? r.a : T(0);
+  q.q = lane == kWave - 1 ? r.c : T(0);
+  for (int s = 1; s < kWave; s <<= 1) {
+    const Row3<T> m = row3_from_lane(q, lane - s, lane >= s);
+    const Row3<T> n = row3_from_lane(q, lane + s, lane + s < kWave);
+    q = pcr_update3(q, m, n);
+  }
+  if (lane == 0) {
+    bs[6 * w] = q.d;
+    bs[6 * w + 1] = q.p;
+    bs[6 * w + 2] = q.q;
+  }
+  if (lane == kWave - 1) {
+    bs[6 * w + 3] = q.d;
+    bs[6 * w + 4] = q.p;
+    bs[6 * w + 5] = q.q;
+  }
+  __syncthreads();
+  if (w == 0 && lane == 0) {
+    // z_v = (first_v, last_v):  z_v + B_v z_{v-1} + C_v z_{v+1} = (D0, D63)
+    // with B_v = [[0, P0], [0, P63]], C_v = [[Q0, 0], [Q63, 0]];
+    // z_v = H_v - G_v z_{v+1}, G_v = [[G0, 0], [G1, 0]]
+    T g1 = T(0), h1 = T(0);
+    for (int v = 0; v < W; v++) {
+      T *e = bs + 6 * v;
+      const T D0 = e[0], P0 = e[1], Q0 = e[2], D63 = e[3], P63 = e[4], Q63 = e[5];
+      const T inv = rcp(T(1) - P0 * g1);
+      const T H0 = (D0 - P0 * h1) * inv;
+      const T H1 = D63 - P63 * h1 + P63 * g1 * H0;
+      const T G0 = Q0 * inv;
+      const T G1 = Q63 + P63 * g1 * G0;
+      e[0] = G0;
+      e[1] = G1;
+      e[2] = H0;
+      e[3] = H1;
+      g1 = G1;
+      h1 = H1;
+    }
+    T xn = T(0);  // first l of the next wave
+    for (int v = W - 1; v >= 0; v--) {
+      const T *e = bs + 6 * v;
+      const T first = e[2] - e[0] * xn;
+      zs[2 * v] = first;
+      zs[2 * v + 1] = e[3] - e[1] * xn;
+      xn = first;
+    }
+  }
+  __syncthreads();
+  const T xp = w > 0 ? zs[2 * w - 1] : T(0);
+  const T xnx = w + 1 < W ? zs[2 * w + 2] : T(0);
+  xl = q.d - q.p * xp - q.q * xnx;
+  T xm = lane_read(xl, lane > 0 ? lane - 1 : lane);
+  if (lane == 0) xm = xp;
+  xf = f.d - f.a * xm - f.c * xl;
 }
 
 // Streaming (non-temporal) global accesses: every element of a, b, c, d
@@ -756,8 +810,8 @@ struct CoefKappa {
 //            host).  2: wave-staged access (ORDER 0, cps <= 64, every
 //            system exactly cps*R long and the systems of a wave adjacent in
 //            memory; checked on the host).  3: the same for cps > 64 (each
-//            wave a contiguous span of one system), with the two-level
-//            reduced solve (reduced_segs).
+//            wave a contiguous span of one system), with the wave-blocked
+//            reduced solve (reduced_multi).
 //   I:       index type of element offsets (uint32_t or long long).
 // blockDim.x == cps * spb, both powers of two.  Non-Inc: d <- A^{-1} d;
 // Inc: u <- u + A^{-1} d.
@@ -1085,9 +1139,9 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
   if (ORDER == 0 && cps <= kWave) {
     reduced_wave<T>(f, l, k, cps, tid & (kWave - 1), xf, xl);
   } else if (ORDER == 0 && (VEC == 3 || VEC == 4)) {
-    // segment rows after the wave staging buffers
-    reduced_segs<T>(f, l, k, cps, slot,
-                    reinterpret_cast<T *>(smem_raw) + (size_t)(blockDim.x >> 6) * SW, xf, xl);
+    // wave rows after the wave staging buffers
+    reduced_multi<T>(f, l, k, cps, slot,
+                     reinterpret_cast<T *>(smem_raw) + (size_t)(blockDim.x >> 6) * SW, xf, xl);
   } else if (cps <= kWave) {
     reduced_lds_w<T>(f, l, k, cps, slot, spb, reinterpret_cast<T *>(smem_raw), xf, xl);
   } else {
