@@ -33,7 +33,8 @@ sample.  "extras" (same line) holds the other measured workloads:
             five in one mixed-M launch;
   split     systems longer than one work-group (split-line solves): 8
             systems of 2^22 rows (solvedim 0) and 64 systems of 2^19 rows
-            (solvedim 1), FP64 and FP32.
+            (solvedim 1), FP64 and FP32;
+  adi_sizes the ADI step at 256^3 FP32 and 512^3 FP64 / FP32.
 """
 from __future__ import annotations
 
@@ -62,8 +63,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
-    ap.add_argument("--extras", default="plain,ragged,vadi,block,split",
-                    help="comma-separated subset of plain,ragged,vadi,block")
+    ap.add_argument("--extras", default="plain,ragged,vadi,block,split,adi_sizes",
+                    help="comma-separated subset of plain,ragged,vadi,block,split,adi_sizes")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (profiler calibration)")
     return ap.parse_args()
@@ -335,6 +336,54 @@ def extra_split(torch, np, ta, reps=10):
     return out
 
 
+def extra_adi_sizes(torch, np, ta, steps=10):
+    """The ADI time step at the other grid sizes and precisions of the ADI
+    configuration: 256^3 FP32, 512^3 FP64 and FP32 (autotuned plans, eager
+    steps and graph replay)."""
+    from tridsolver_amd.adi import AdiPlan
+    out = {"workload": "ADI heat diffusion as the headline (lambda=1, a=c=-lambda, "
+                       "b=1+2lambda, Dirichlet-zero, u0~U(0,1) seed 0), other grids",
+           "bytes_model": "9 words per point per step (rhs: u in, du out; x, y: du in/out; "
+                          "z: du, u in, u out)"}
+    for n, dname in ((256, "f32"), (512, "f64"), (512, "f32")):
+        tdt = torch.float64 if dname == "f64" else torch.float32
+        ws = 8 if dname == "f64" else 4
+        u = torch.zeros((n, n, n), dtype=tdt, device="cuda")
+        g = torch.Generator(device="cuda").manual_seed(0)
+        u[1:-1, 1:-1, 1:-1] = torch.rand((n - 2, n - 2, n - 2), generator=g, device="cuda",
+                                         dtype=tdt)
+        du = torch.empty_like(u)
+        plan = AdiPlan(n, n, n, tdt, 1.0, 1)
+        plan.step(u, du, 3)  # tunes at first use
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        plan.step(u, du, steps, sync=0)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        plan.capture(u, du, steps)
+        plan.replay()
+        torch.cuda.synchronize()
+        e0.record()
+        plan.replay(sync=0)
+        e1.record()
+        torch.cuda.synchronize()
+        gms = e0.elapsed_time(e1) / steps
+        pts = n ** 3
+        out[f"{n}^3_{dname}"] = {
+            "ms_per_step": round(ms, 4), "graph_ms_per_step": round(gms, 4),
+            "Gunknowns_s": round(3 * pts / (ms * 1e-3) / 1e9, 1),
+            "graph_Gunknowns_s": round(3 * pts / (gms * 1e-3) / 1e9, 1),
+            "frac_9words": round(9 * ws * pts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "config": plan.describe()}
+        plan.close()
+        del u, du
+        torch.cuda.empty_cache()
+    return out
+
+
 def extra_ragged(torch, np, ta, reps=10):
     import ctypes as C
     from tridsolver_amd.ragged import RaggedPlan
@@ -589,7 +638,7 @@ def main():
         del W
         torch.cuda.empty_cache()
         fns = {"plain": extra_plain, "ragged": extra_ragged, "vadi": extra_vadi,
-               "block": extra_block, "split": extra_split}
+               "block": extra_block, "split": extra_split, "adi_sizes": extra_adi_sizes}
         out["extras"] = {}
         for k in args.extras.split(","):
             k = k.strip()

@@ -82,8 +82,10 @@ def test_adi_reference_mode_vs_reference_gold(name):
 # constant-coefficient factors); (6, 600, 5) and (5, 6, 700) exercise the
 # generic sweep kernels along y and z; short lines (3, 5, 9 rows) the
 # single-chunk and identity-padded factor layouts
+# 512-point lines along each axis: the 512^3 configuration's kernels
 ADI_SHAPES = [(32, 32, 32), (33, 17, 9), (64, 40, 24), (8, 200, 6), (128, 64, 48),
-              (6, 600, 5), (5, 6, 700), (16, 3, 5), (20, 9, 3)]
+              (6, 600, 5), (5, 6, 700), (16, 3, 5), (20, 9, 3),
+              (512, 24, 20), (20, 512, 24), (24, 20, 512)]
 
 
 @pytest.mark.parametrize("shape", ADI_SHAPES)
@@ -201,7 +203,8 @@ def test_adi_graph_replays_match_eager():
         np.testing.assert_array_equal(ug.cpu().numpy(), ue.cpu().numpy(), err_msg=f"replay {rep}")
 
 
-@pytest.mark.parametrize("shape", [(64, 40, 24), (128, 64, 48), (6, 600, 5)])
+@pytest.mark.parametrize("shape", [(64, 40, 24), (128, 64, 48), (6, 600, 5), (512, 24, 20),
+                                   (24, 20, 512)])
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 def test_adi_autotuned_plan_vs_oracle(shape, dt):
     """With autotuning on, the plan times its kernel choices at its first use on
