@@ -359,6 +359,14 @@ inline PartCfg split_cfg(int order, int vec, size_t tsize, long long nsys) {
 }
 
 inline PartCfg heuristic(int order, int N, int vec, size_t tsize) {
+  if (order == 1 && N > 512 && N <= 1024) {
+    // strided systems of 513..1024 rows: 16-row chunks keep 8 systems per
+    // 512-thread group (8-row chunks leave 4), in the XCD-contiguous group
+    // order (15 % -> 40-48 % of HBM bandwidth on MI355X)
+    PartCfg c = make_cfg(order, 16, N, 0, 512);
+    c.xcd = 1;
+    if (cfg_valid(c, N)) return c;
+  }
   // preferred chunk lengths: FP64 8 rows, FP32 16 rows (register budget);
   // FP32 along contiguous rows with 16-byte loads: 4 rows, one load per
   // lane and array, consecutive lanes on consecutive 16 bytes (fastest at

@@ -480,7 +480,13 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
       }
     }
   }
-  // systems longer than one work-group holds: split-line solve
+  // systems longer than one work-group holds: split-line solve.  Strided
+  // systems (ORDER 1) of 2048 rows and more take it too: a work-group that
+  // holds whole systems of that length has room for one or two of them,
+  // i.e. 8- to 16-byte row runs (3-30 % of HBM bandwidth on MI355X), while
+  // the split-line segments keep 64 systems side by side (about 69 % of its
+  // 9 words per unknown, i.e. 38 % on the one-pass scale)
+  if (fc.kind == 0 && cfg.kind == 1 && order == 1 && D.N >= 2048) cfg.kind = 4;
   if (cfg.kind == 4 && fc.kind != 4) cfg = split_cfg(order, vec, sizeof(T), D.nsys);
   if (cfg.kind == 4 && D.nsys > (1LL << 30))
     return fail(TRID_STATUS_INVALID_VALUE, "split-line solve: too many systems");
