@@ -102,7 +102,7 @@ def plain_workload(args, torch, np, api, ta):
     unknowns = n ** 3
     # algorithmic bytes of one solve: read a, b, c, d and write d
     algo_bytes = 5 * wsize * unknowns
-    configs = {}
+    kernel_cfg = {}
 
     def step(events=None):
         for sd in (0, 1, 2):
@@ -113,11 +113,11 @@ def plain_workload(args, torch, np, api, ta):
                 raise RuntimeError(f"solve failed: {st} {ta.last_error()}")
             if events is not None:
                 events[sd][1].record()
-            configs[sd] = ta.last_config()
+            kernel_cfg[sd] = ta.last_config()
 
     return dict(step=step, kernels=("solvedim0", "solvedim1", "solvedim2"),
                 unknowns_per_step=3 * unknowns, algo_bytes=[algo_bytes] * 3,
-                host=host, dims=dims, configs=configs,
+                host=host, dims=dims, kernel_cfg=kernel_cfg,
                 workload=f"plain batched scalar solve {n}^3 along solvedim 0,1,2")
 
 
@@ -168,7 +168,7 @@ def adi_workload(args, torch, np, api, ta):
 
     return dict(step=step, kernels=names,
                 unknowns_per_step=3 * pts, algo_bytes=algo, host=[u0], dims=[n, n, n],
-                configs={}, describe=plan.describe, graph_ms=graph_ms,
+                kernel_cfg={}, describe=plan.describe, graph_ms=graph_ms,
                 workload=f"ADI heat diffusion {n}^3, lambda=1, a=c=-lambda, b=1+2lambda, "
                          f"Dirichlet-zero, u0~U(0,1) seed 0 (apps/adi time step)")
 
@@ -571,7 +571,7 @@ def main():
         W["step"]()
     torch.cuda.synchronize()
     if "describe" in W:  # the plan tunes at its first use
-        W["configs"] = {k: W["describe"]() for k in range(len(W["kernels"]))}
+        W["kernel_cfg"] = {k: W["describe"]() for k in range(len(W["kernels"]))}
 
     nk = len(W["kernels"])
     ev = [[[torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)]
@@ -601,7 +601,7 @@ def main():
             "avg_ms": round(avg[k], 5),
             "GB_s": round(W["algo_bytes"][k] / (avg[k] * 1e-3) / 1e9, 1),
             "frac": round(W["algo_bytes"][k] / (avg[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "config": W["configs"].get(k, ""),
+            "config": W["kernel_cfg"].get(k, ""),
         }
     out = {
         "metric": METRIC,
