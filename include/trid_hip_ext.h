@@ -48,6 +48,13 @@ EXTERN_C const char *tridGetLastConfig(void);
  * (0 = automatic). */
 EXTERN_C tridStatus_t tridForceConfig(int kind, int rows, int cps, int spb);
 
+/* Diagnostic: on = 1 makes every later call use 64-bit element offsets and
+ * 64-bit system-index arithmetic, the forms the kernels take for arrays of
+ * 4 GiB or more and batches of 2^31 systems or more, whatever the sizes (so
+ * that tests reach those forms on small arrays); on = 0 (the default)
+ * restores the automatic choice. */
+EXTERN_C tridStatus_t tridForceWideIndex(int on);
+
 /* Diagnostic: y[i] = the solvers' reciprocal of x[i] for n device values
  * (double when is_double, else float).  The solvers form every pivot
  * reciprocal with this routine; tests check its error bound. */

@@ -77,6 +77,12 @@ def force_config(kind: int = 0, rows: int = 0, cps: int = 0, spb: int = 0) -> No
     _check(lib().tridForceConfig(kind, rows, cps, spb), "tridForceConfig")
 
 
+def force_wide_index(on: bool) -> None:
+    """Makes later calls use the 64-bit offset forms of the kernels (taken
+    automatically for arrays of 4 GiB or more) whatever the sizes."""
+    _check(lib().tridForceWideIndex(1 if on else 0), "tridForceWideIndex")
+
+
 def last_config() -> str:
     return lib().tridGetLastConfig().decode()
 

@@ -29,6 +29,7 @@ SIGNATURES = {
     "tridGetStatusString": (C.c_char_p, [I]),
     "tridGetLastConfig": (C.c_char_p, []),
     "tridForceConfig": (ST, [I, I, I, I]),
+    "tridForceWideIndex": (ST, [I]),
     "tridDebugReciprocal": (ST, [P, P, C.c_longlong, I]),
     # trid_adi.h
     "tridAdiPlanCreate": (ST, [C.POINTER(P), I, I, I, I, C.c_double, I]),

@@ -193,7 +193,7 @@ SysDesc grid_desc(int nx, int ny, int nz, int dir) {
     o++;
   }
   for (int t = 0; t < 5; t++) D.sstr[t] = str[dir];
-  D.small_index = D.nsys < (1LL << 31);
+  D.small_index = D.nsys < (1LL << 31) && !wide_index();
   return D;
 }
 
@@ -678,7 +678,7 @@ EXTERN_C tridStatus_t tridAdiPlanCreate(tridAdiPlan_t *plan, int nx, int ny,
   p->lambda = lambda;
   p->mode = mode;
   const size_t ts = p->dbl ? 8 : 4;
-  const int idx32 = (unsigned long long)nx * ny * nz * ts < (1ULL << 32);
+  const int idx32 = (unsigned long long)nx * ny * nz * ts < (1ULL << 32) && !wide_index();
   const int n[3] = {nx, ny, nz};
   for (int dir = 0; dir < 3; dir++) {
     p->desc[dir] = grid_desc(nx, ny, nz, dir);

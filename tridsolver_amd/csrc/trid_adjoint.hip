@@ -80,13 +80,13 @@ tridStatus_t strided_adjoint(const T *a, const T *b, const T *c, const T *x, con
   if (D.N == 0 || D.nsys == 0) return TRID_STATUS_SUCCESS;
   if (!a || !b || !c || !x || !g || !ga || !gb || !gc || !gd)
     return fail(TRID_STATUS_INVALID_VALUE, "NULL array");
-  D.small_index = D.nsys < (1LL << 31);
+  D.small_index = D.nsys < (1LL << 31) && !wide_index();
   hipStream_t st = current_stream();
   const int order = solvedim == 0 ? 0 : 1;
   PartCfg cfg = heuristic(order, D.N, 0, sizeof(T));
   if (cfg.kind != 1) return fail(TRID_STATUS_INVALID_VALUE, "system too long for the adjoint kernel");
   cfg.vec = 0;
-  cfg.idx32 = (unsigned long long)ext * sizeof(T) < (1ULL << 32);
+  cfg.idx32 = (unsigned long long)ext * sizeof(T) < (1ULL << 32) && !wide_index();
   // lambda = A^T^{-1} g, written to grad_d (g itself is not modified)
   const CoefTransposed<T> cf{a, b, c};
   tridStatus_t s = check_hip(launch_part<T, 0>(cfg, cf, const_cast<T *>(g), nullptr, D, st, gd),

@@ -16,6 +16,7 @@ namespace tridhip {
 namespace {
 std::atomic<hipStream_t> g_stream{nullptr};
 std::atomic<int> g_autotune{-1};
+std::atomic<int> g_wide{0};
 std::mutex g_mu;
 ForcedConfig g_forced;
 thread_local std::string t_error;
@@ -80,6 +81,8 @@ int autotune_mode() {
   return m;
 }
 
+bool wide_index() { return g_wide.load() != 0; }
+
 ForcedConfig forced_config() {
   std::lock_guard<std::mutex> lk(g_mu);
   return g_forced;
@@ -135,5 +138,11 @@ EXTERN_C tridStatus_t tridForceConfig(int kind, int rows, int cps, int spb) {
   g_forced.rows = rows;
   g_forced.cps = cps;
   g_forced.spb = spb;
+  return TRID_STATUS_SUCCESS;
+}
+
+EXTERN_C tridStatus_t tridForceWideIndex(int on) {
+  if (on != 0 && on != 1) return fail(TRID_STATUS_INVALID_VALUE, "tridForceWideIndex: on must be 0 or 1");
+  g_wide.store(on);
   return TRID_STATUS_SUCCESS;
 }

@@ -27,6 +27,10 @@ void *workspace(size_t bytes);
 
 int autotune_mode();
 
+// True while tridForceWideIndex(1) is in effect: the kernels then use 64-bit
+// element offsets and 64-bit system-index arithmetic whatever the sizes.
+bool wide_index();
+
 struct ForcedConfig {
   int kind = 0, rows = 0, cps = 0, spb = 0, stage = 1;
 };

@@ -182,7 +182,7 @@ hipError_t split_solve(const PartCfg &cfg, const T *a, const T *b, const T *c, T
     for (int t = 0; t < 5; t++) Ds.ostr[t][slot] = (long long)L * D.sstr[t];
     Ds.nother = D.nother + 1;
     Ds.nsys = ns * Pf;
-    Ds.small_index = Ds.nsys < (1LL << 31);
+    Ds.small_index = Ds.nsys < (1LL << 31) && !wide_index();
   }
   PartCfg cs = cfg;
   cs.kind = 1;
@@ -232,8 +232,8 @@ hipError_t split_solve(const PartCfg &cfg, const T *a, const T *b, const T *c, T
     Dr.sstr[t] = so.rs_row;
     Dr.ostr[t][0] = so.rs_sys;
   }
-  Dr.small_index = 1;
-  const int ridx32 = (unsigned long long)n2 * sizeof(T) < (1ULL << 32);
+  Dr.small_index = !wide_index();
+  const int ridx32 = (unsigned long long)n2 * sizeof(T) < (1ULL << 32) && !wide_index();
   // (16-byte loads along the reduced rows for ORDER 0: the workspace arrays
   // are 16-byte aligned and each system's 2P rows start at a multiple of 2P)
   const int rvec = order == 0 && (2 * P * (int)sizeof(T)) % 16 == 0;
@@ -379,9 +379,9 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
       cum *= pads[t][k];
     }
   }
-  D.small_index = D.nsys < (1LL << 31);
+  D.small_index = D.nsys < (1LL << 31) && !wide_index();
   // 32-bit offsets when every array (its allocated extent) fits in 4 GiB
-  int idx32 = 1;
+  int idx32 = !wide_index();
   for (int t = 0; t < 5; t++) {
     if (t == 4 && !INC) break;
     if ((unsigned long long)extent(ndim, dims, pads[t]) * sizeof(T) >= (1ULL << 32))

@@ -365,7 +365,7 @@ SysDesc line_desc(const Grid &g, int axis) {
     o++;
   }
   for (int t = 0; t < 5; t++) D.sstr[t] = str[axis];
-  D.small_index = D.nsys < (1LL << 31);
+  D.small_index = D.nsys < (1LL << 31) && !wide_index();
   return D;
 }
 
@@ -379,7 +379,7 @@ tridStatus_t vadi_sweep(const double *K, double *d, double *u, double *out, cons
   PartCfg cfg = heuristic(order, D.N, 0, sizeof(double));
   if (cfg.kind != 1) return fail(TRID_STATUS_INVALID_VALUE, "vadi: grid too long along an axis");
   cfg.vec = 0;
-  cfg.idx32 = (unsigned long long)g.n * 8 < (1ULL << 32);
+  cfg.idx32 = (unsigned long long)g.n * 8 < (1ULL << 32) && !wide_index();
   const CoefKappa<double> cf{K, lam, transpose};
   hipError_t e = inc ? launch_part<double, 1>(cfg, cf, d, u, D, st, out)
                      : launch_part<double, 0>(cfg, cf, d, u, D, st, out);
