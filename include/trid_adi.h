@@ -57,4 +57,16 @@ EXTERN_C tridStatus_t tridAdiGraphLaunch(tridAdiPlan_t plan, int sync);
 /* Describes the kernel configurations of the plan's four stages. */
 EXTERN_C const char *tridAdiPlanDescribe(tridAdiPlan_t plan);
 
+/* Pins the y (dir = 1) or z (dir = 2) sweep of the plan to one kernel
+ * layout, for tests and reproducible timings.  rows = 8, 16 or 32 selects
+ * the sweep with precomputed factors and chunks of that many rows (pair = 1:
+ * two adjacent lines per lane where the arrays allow it; xcd = 1: every XCD
+ * takes one contiguous range of lines); rows = 0 the partitioned sweep
+ * without precomputed factors.  Returns TRID_STATUS_INVALID_VALUE when the
+ * layout does not fit the line length.  Pending autotuning of the plan is
+ * cancelled; a graph captured earlier keeps the kernels it was captured
+ * with. */
+EXTERN_C tridStatus_t tridAdiPlanSetSweep(tridAdiPlan_t plan, int dir, int rows, int pair,
+                                          int xcd);
+
 #endif

@@ -6,6 +6,8 @@
 * both modes against the oracle's ADI at several grid shapes and at the
   benchmark size 256^3 (FP64 and FP32, 20 steps);
 * each stage against the oracle (the right-hand side bit-exactly);
+* every y / z sweep layout (tridAdiPlanSetSweep) against the oracle, and the
+  layouts that do not fit a line length rejected with a status;
 * HIP-graph replays against eager execution, several replays with eager
   work and host synchronisation in between.
 Tolerance: normwise relative error 1e-12 (FP64), 1e-5 (FP32).
@@ -19,6 +21,7 @@ import numpy as np
 import pytest
 
 import oracle
+import tridsolver_amd as ta
 from tridsolver_amd.adi import AdiPlan
 
 pytestmark = pytest.mark.gpu
@@ -113,6 +116,42 @@ def test_adi_benchmark_size(dt):
     err = oracle.rel_err(exp, got.ravel())
     print(f"ADI 256^3 {np.dtype(dt).name} 20 steps: rel err vs oracle {err:.2e} [{plan.describe()}]")
     assert err <= tol(dt)
+
+
+# every y / z sweep layout: rows 0 (partitioned sweep) or 8/16/32 (precomputed
+# factors) x two lines per lane x XCD order, on lines of 24..512 rows
+SWEEP_LAYOUTS = [(0, 0, 0)] + [(r, p, x) for r in (8, 16, 32) for p in (0, 1) for x in (0, 1)]
+
+
+@pytest.mark.parametrize("shape", [(64, 40, 24), (24, 256, 130), (16, 20, 512), (10, 9, 7)])
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_adi_every_sweep_layout(shape, dt):
+    import torch
+    nx, ny, nz = shape
+    u0 = rand_init(nx, ny, nz, dt, seed=nx * ny)
+    exp = u0.ravel().copy()
+    assert oracle.adi(exp, nx, ny, nz, 3, 1.0, 1) == 0
+    ran = 0
+    for direction in (1, 2):
+        n = ny if direction == 1 else nz
+        for rows, pair, xcd in SWEEP_LAYOUTS:
+            plan = AdiPlan(nx, ny, nz, torch_dt(dt), 1.0, 1)
+            cps = -(-n // rows) if rows else 1
+            cps = 1 << (cps - 1).bit_length()
+            fits = rows == 0 or (64 * cps <= 1024 and (not pair or (rows <= 32 and
+                                                                  64 * cps <= (512 if rows == 32 else 1024))))
+            if not fits:
+                with pytest.raises(ta.TridError):
+                    plan.set_sweep(direction, rows, pair, xcd)
+                continue
+            plan.set_sweep(direction, rows, pair, xcd)
+            u = torch.from_numpy(u0.copy()).to("cuda")
+            plan.step(u, torch.empty_like(u), 3)
+            what = f"{shape} dir {direction} layout {(rows, pair, xcd)} [{plan.describe()}]"
+            err = oracle.rel_err(exp, u.cpu().numpy().ravel())
+            assert err <= tol(dt), f"{what}: {err:.2e}"
+            ran += 1
+    print(f"{shape} {dt.__name__}: {ran} layouts")
 
 
 def test_adi_stages_vs_oracle():

@@ -39,6 +39,7 @@ SIGNATURES = {
     "tridAdiGraphCapture": (ST, [P, P, P, I]),
     "tridAdiGraphLaunch": (ST, [P, I]),
     "tridAdiPlanDescribe": (C.c_char_p, [P]),
+    "tridAdiPlanSetSweep": (ST, [P, I, I, I, I]),
     # blktrid_hip.h
     "sbtsv_gpu": (None, [P, P, P, P, P, P, P, I, I, I]),
     "dbtsv_gpu": (None, [P, P, P, P, P, P, P, I, I, I]),

@@ -49,6 +49,12 @@ class AdiPlan:
     def describe(self) -> str:
         return lib().tridAdiPlanDescribe(self._h).decode()
 
+    def set_sweep(self, direction: int, rows: int, pair: int = 0, xcd: int = 0) -> None:
+        """Pins the y (1) or z (2) sweep: rows 8/16/32 = precomputed factors
+        with that chunk length, 0 = the partitioned sweep."""
+        _check(lib().tridAdiPlanSetSweep(self._h, int(direction), int(rows), int(pair), int(xcd)),
+               "tridAdiPlanSetSweep")
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             lib().tridAdiPlanDestroy(self._h)

@@ -259,9 +259,15 @@ bool cc_rows_ok(int N, int R, int *cps_log2) {
   const long long ch = (N + R - 1) / R;
   int cl = 0;
   while ((1LL << cl) < ch) cl++;
-  if (cl > 4) return false;
+  if ((kWave << cl) > cc_max_threads(R, 1)) return false;
   *cps_log2 = cl;
   return true;
+}
+
+// Two systems per lane fit chunk lengths up to 32 rows in work-groups within
+// the kernel's launch bound.
+bool cc_pair_fits(int R, int cps_log2) {
+  return R <= 32 && (kWave << cps_log2) <= cc_max_threads(R, 2);
 }
 
 // Builds the factor table of direction dir for chunk length R into *F
@@ -278,16 +284,16 @@ tridStatus_t cc_build(tridAdiPlan_st *p, int dir, int R, void **F, int *cps_log2
   tridStatus_t s = check_hip(hipMalloc(F, cps * nf * sizeof(T)), "hipMalloc");
   if (s) return s;
   const CoefConst<T> cf = coefs<T>(p);
-  const int threads = std::max(cps, kWave);
+  // one wave (cps <= 16)
   hipStream_t st = current_stream();
   if (R == 8)
-    hipLaunchKernelGGL((k_cc_setup<T, 8>), dim3(1), dim3(threads), 0, st, cf, N, *cps_log2,
+    hipLaunchKernelGGL((k_cc_setup<T, 8>), dim3(1), dim3(kWave), 0, st, cf, N, *cps_log2,
                        static_cast<T *>(*F));
   else if (R == 16)
-    hipLaunchKernelGGL((k_cc_setup<T, 16>), dim3(1), dim3(threads), 0, st, cf, N, *cps_log2,
+    hipLaunchKernelGGL((k_cc_setup<T, 16>), dim3(1), dim3(kWave), 0, st, cf, N, *cps_log2,
                        static_cast<T *>(*F));
   else
-    hipLaunchKernelGGL((k_cc_setup<T, 32>), dim3(1), dim3(threads), 0, st, cf, N, *cps_log2,
+    hipLaunchKernelGGL((k_cc_setup<T, 32>), dim3(1), dim3(kWave), 0, st, cf, N, *cps_log2,
                        static_cast<T *>(*F));
   s = check_hip(hipGetLastError(), "factor setup launch");
   if (!s) s = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
@@ -325,19 +331,20 @@ hipError_t launch_cc_t(int R, int cps_log2, int xcd, int pair, const void *F, co
   if (R == 8) TRID_CC(8, II, NN);       \
   else if (R == 16) TRID_CC(16, II, NN); \
   else TRID_CC(32, II, NN);
-  // pairs: chunk lengths 8 and 16 only (32 rows of two systems exceed the
-  // register budget of 1024-thread work-groups)
+  // pairs: chunk lengths 8, 16 and 32 (cc_pair_fits)
   if (idx32) {
     if (pair) {
       if (R == 8) TRID_CC(8, uint32_t, 2);
-      else TRID_CC(16, uint32_t, 2);
+      else if (R == 16) TRID_CC(16, uint32_t, 2);
+      else TRID_CC(32, uint32_t, 2);
     } else {
       TRID_CC_R(uint32_t, 1)
     }
   } else {
     if (pair) {
       if (R == 8) TRID_CC(8, long long, 2);
-      else TRID_CC(16, long long, 2);
+      else if (R == 16) TRID_CC(16, long long, 2);
+      else TRID_CC(32, long long, 2);
     } else {
       TRID_CC_R(long long, 1)
     }
@@ -349,7 +356,8 @@ hipError_t launch_cc_t(int R, int cps_log2, int xcd, int pair, const void *F, co
 
 template <typename T, int INC>
 hipError_t launch_cc(const tridAdiPlan_st *p, int dir, T *d, T *u, hipStream_t st) {
-  const int pair = p->cc_pair[dir] && p->cc_R[dir] <= 16 && cc_pair_ok<T>(p->desc[dir], d, u, INC);
+  const int pair = p->cc_pair[dir] && cc_pair_fits(p->cc_R[dir], p->cc_cps_log2[dir]) &&
+                   cc_pair_ok<T>(p->desc[dir], d, u, INC);
   return launch_cc_t<T, INC>(p->cc_R[dir], p->cc_cps_log2[dir], p->cc_xcd[dir], pair,
                              p->cc_F[dir], p->desc[dir], p->cfg[dir].idx32, d, u, st);
 }
@@ -545,7 +553,7 @@ void tune_plan_t(tridAdiPlan_st *p, T *u, T *du) {
       if (!cc_rows_ok(n[dir], R, &cl)) continue;
       if (cc_build<T>(p, dir, R, &F, &cl) != TRID_STATUS_SUCCESS) continue;
       tables.push_back(F);
-      for (int pr = 0; pr <= (pair_ok && R <= 16 ? 1 : 0); pr++) {
+      for (int pr = 0; pr <= (pair_ok && cc_pair_fits(R, cl) ? 1 : 0); pr++) {
         ch.push_back({1, R, cl, 0, pr, p->cfg[dir], F});
         ch.push_back({1, R, cl, 1, pr, p->cfg[dir], F});
       }
@@ -718,7 +726,7 @@ EXTERN_C tridStatus_t tridAdiPlanCreate(tridAdiPlan_t *plan, int nx, int ny,
       if (r == TRID_STATUS_SUCCESS) {
         p->cc_R[dir] = R;
         p->cc_cps_log2[dir] = cl;
-        p->cc_pair[dir] = R <= 16;  // used when the arrays allow it (cc_pair_ok)
+        p->cc_pair[dir] = cc_pair_fits(R, cl);  // used when the arrays allow it (cc_pair_ok)
         p->cc_F[dir] = F;
       }
       break;
@@ -816,4 +824,36 @@ EXTERN_C tridStatus_t tridAdiGraphLaunch(tridAdiPlan_t p, int sync) {
 
 EXTERN_C const char *tridAdiPlanDescribe(tridAdiPlan_t p) {
   return p ? p->text.c_str() : "";
+}
+
+EXTERN_C tridStatus_t tridAdiPlanSetSweep(tridAdiPlan_t p, int dir, int rows, int pair,
+                                          int xcd) {
+  if (!p) return fail(TRID_STATUS_INVALID_VALUE, "NULL plan");
+  if (dir != 1 && dir != 2) return fail(TRID_STATUS_INVALID_VALUE, "dir must be 1 (y) or 2 (z)");
+  if ((pair != 0 && pair != 1) || (xcd != 0 && xcd != 1))
+    return fail(TRID_STATUS_INVALID_VALUE, "pair and xcd must be 0 or 1");
+  void *F = nullptr;
+  int cl = 0;
+  if (rows != 0) {
+    if (!cc_rows_ok(p->desc[dir].N, rows, &cl))
+      return fail(TRID_STATUS_INVALID_VALUE, "rows do not fit the line length");
+    if (pair && !cc_pair_fits(rows, cl))
+      return fail(TRID_STATUS_INVALID_VALUE, "two lines per lane do not fit these rows");
+    const tridStatus_t s = p->dbl ? cc_build<double>(p, dir, rows, &F, &cl)
+                                  : cc_build<float>(p, dir, rows, &F, &cl);
+    if (s) return s;
+  } else if (pair || xcd) {
+    return fail(TRID_STATUS_INVALID_VALUE, "pair and xcd apply to rows 8, 16 and 32");
+  }
+  // hipFree waits for queued kernels that may still read the old factors
+  if (p->cc_F[dir]) (void)hipFree(p->cc_F[dir]);
+  p->cc_F[dir] = F;
+  p->cc_R[dir] = rows;
+  p->cc_cps_log2[dir] = cl;
+  p->cc_pair[dir] = pair;
+  p->cc_xcd[dir] = xcd;
+  p->tune_pending = false;
+  p->text = plan_text(p);
+  clear_error();
+  return TRID_STATUS_SUCCESS;
 }

@@ -27,6 +27,13 @@
 
 namespace tridhip {
 
+// Largest work-group of k_cc_sweep for chunk length R and NS systems per
+// lane (its launch bound): 64 chunk rows per lane (R = 32 with two systems)
+// need more than the 128 VGPRs of a 1024-thread work-group.
+__host__ __device__ constexpr int cc_max_threads(int R, int NS) {
+  return R * NS >= 64 ? 512 : 1024;
+}
+
 // Per-chunk factor record (T values) for chunk length R.
 template <int R>
 struct CCLayout {
@@ -51,14 +58,13 @@ struct CCLayout {
 // Factors of the interior-line matrix of a constant-coefficient batch (rows
 // interior to the grid when 1 <= row <= N-2: a, b, c = cf; identity rows
 // otherwise), for chunk length R and cps = 2^cps_log2 chunks.  One
-// work-group of max(cps, 64) threads; thread k writes chunk k's record at
-// F + k * nf.  The recurrences are those of chunk_reduce, eliminate_first
+// wave (cps <= 64); thread k writes chunk k's record at F + k * nf.  The recurrences are those of chunk_reduce, eliminate_first
 // and pcr_update, applied to the coefficients only.
 template <typename T, int R>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(kWave)
     k_cc_setup(CoefConst<T> cf, int N, int cps_log2, T *__restrict__ F) {
   using Ly = CCLayout<R>;
-  __shared__ T sa[1024], sc[1024];
+  __shared__ T sa[kWave], sc[kWave];
   const int cps = 1 << cps_log2;
   const int nf = Ly::nf(cps_log2);
   const int k = threadIdx.x;
@@ -148,14 +154,14 @@ __global__ void __launch_bounds__(1024)
 }
 
 // The solve with precomputed factors F (k_cc_setup of the same N, R, cps).
-// blockDim = 64 * cps; wave w = chunk w; lane l of work-group b takes NS
+// blockDim = 64 * cps <= cc_max_threads(R, NS); wave w = chunk w; lane l of work-group b takes NS
 // adjacent systems q = NS * (64 * b + l) + s, s < NS.  NS = 2 moves both
 // systems' rows as one 16-byte access (the host requires adjacent systems to
 // be adjacent elements of d and u, 16-byte aligned, and an even system
 // count), so a wave's row access covers 128 systems without gaps.
 // Non-Inc: d <- A^{-1} d; Inc: u <- u + A^{-1} d.
 template <typename T, int INC, int R, typename I, int NS>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(cc_max_threads(R, NS))
     k_cc_sweep(const T *__restrict__ F, T *__restrict__ Dd, T *__restrict__ U, SysDesc D,
                int cps_log2, int xcd) {
   using Ly = CCLayout<R>;
