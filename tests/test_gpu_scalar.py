@@ -158,6 +158,23 @@ def test_every_kernel_configuration(cfg, sd, dtype):
         ta.force_config(0, 0, 0, 0)
 
 
+@pytest.mark.parametrize("sd", [1, 2])
+def test_fp32_wide_strided_groups(sd):
+    """FP32 16-row chunks of strided systems in 1024-thread work-groups (64
+    systems side by side); FP64 refuses that shape."""
+    dims = [70, 200, 9] if sd == 1 else [70, 9, 200]  # 16 chunks x 64 systems
+    ta.force_config(1, 16, 0, 64)
+    try:
+        for inc in (False, True):
+            run_case(dims, sd, np.float32, inc, seed=8)
+            assert "R16,cps16,spb64" in ta.last_config(), ta.last_config()
+        run_case([70, 33, 21], sd, np.float32, False, seed=9)  # 4 chunks x 64 systems
+        with pytest.raises(AssertionError, match="forced configuration"):
+            run_case(dims, sd, np.float64, False)
+    finally:
+        ta.force_config(0, 0, 0, 0)
+
+
 @pytest.mark.parametrize("R", [4, 8, 16])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_staged_multiwave_systems(R, dtype):

@@ -184,18 +184,22 @@ hipError_t launch_seg(const PartCfg &cfg, const CoefArrays<T> &coef, T *d, T *u,
   return hipErrorInvalidValue;
 }
 
-inline bool cfg_valid(const PartCfg &cfg, int N) {
+// tsize: element size; FP32 16-row chunks of strided systems (ORDER 1,
+// unstaged) fit 1024-thread work-groups (79 VGPRs), FP64 ones and staged
+// layouts do not.
+inline bool cfg_valid(const PartCfg &cfg, int N, size_t tsize = 8) {
   if (cfg.kind == 4) {
     // split-line: the segment shape must be a valid work-group shape
     PartCfg s = cfg;
     s.kind = 1;
-    return cfg_valid(s, cfg.R << cfg.cps_log2);
+    return cfg_valid(s, cfg.R << cfg.cps_log2, tsize);
   }
   if (cfg.kind != 1) return true;
   const long long cps = 1LL << cfg.cps_log2, spb = 1LL << cfg.spb_log2;
   const long long thr = cps * spb;
   if (thr < kWave || thr > 1024) return false;
-  if (cfg.R >= 16 && thr > 512) return false;  // register budget of R=16
+  const bool r16_wide = tsize == 4 && cfg.order == 1 && cfg.vec == 0;
+  if (cfg.R >= 16 && thr > 512 && !r16_wide) return false;  // register budget of R=16
   if (cfg.vec == 4 && thr > 512) return false;  // launch bound of general staging
   if (cps * cfg.R < N) return false;
   return true;
@@ -282,7 +286,7 @@ inline std::vector<PartCfg> candidates(int order, int N, int vec, size_t tsize,
   for (int R : Rs)
     for (int t : thr) {
       PartCfg c = make_cfg(order, R, N, vec && (R * (int)tsize) % 16 == 0, t);
-      if (!cfg_valid(c, N)) continue;
+      if (!cfg_valid(c, N, tsize)) continue;
       add_both(c);
       if (stage_ok(c, dense, nsys, N)) {
         c.vec = staged_vec(c);

@@ -438,7 +438,7 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
                    order == 0 ? 256 : 512);
     if (fc.cps) cfg.cps_log2 = ilog2(fc.cps);
     if (fc.spb) cfg.spb_log2 = ilog2(fc.spb);
-    if (!cfg_valid(cfg, D.N))
+    if (!cfg_valid(cfg, D.N, sizeof(T)))
       return fail(TRID_STATUS_INVALID_VALUE, "forced configuration does not fit this shape");
     if (fc.stage && stage_ok(cfg, dense, D.nsys, D.N)) cfg.vec = staged_vec(cfg);
     else if (fc.stage && gstage_ok(cfg, gvec, dense)) cfg.vec = 4;
