@@ -21,7 +21,11 @@
  * item.  Each solve is one
  * launch of a persistent kernel whose work-groups pull items from an atomic
  * counter (reset by the kernel itself).  Systems longer than 4096 rows are
- * solved by a second, one-thread-per-system launch.
+ * solved after it: the longest of them one by one, each as a strided batch
+ * of one system (tridXmtsvStridedBatch, split-line solve past 8192 rows),
+ * the rest by a one-thread-per-system launch; the plan picks the split by a
+ * cost model (a one-system solve ~25 us, the one-thread launch ~0.5 us per
+ * row of its longest system).
  * A plan may be used for any arrays with the same offsets; calls that
  * overlap on one plan race on its counter and must be stream-ordered.
  */

@@ -4,7 +4,8 @@ Each system k of [offsets[k], offsets[k+1]) must equal the reference's 1-D
 solve of that system (oracle_ragged restates trid_scalarD per system),
 within 1e-12 (FP64) / 1e-5 (FP32) normwise, plus the reference tests'
 elementwise tolerance.  Covers every length bin (1..8, wave-level 2..64
-chunks, multi-wave up to 4096, the one-thread path beyond), empty systems,
+chunks, multi-wave up to 4096, and beyond: per-system strided solves for the
+longest systems, the one-thread kernel for the rest), empty systems,
 Inc, repeated solves with one plan (the persistent kernel's self-reset
 counter) and the benchmark configuration (200,000 systems, lengths
 log-uniform in [8, 4096], seed 0).
@@ -94,6 +95,26 @@ def test_ragged_every_chunk_length(dtype):
     check_close(exp, got, dtype, f"ragged [{plan.describe()}]")
     for R in range(5, 9):
         assert f"R{R}/" in plan.describe()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("inc", [False, True])
+def test_ragged_long_systems_both_routes(dtype, inc):
+    """Systems past 4096 rows: the longest solved one by one as strided
+    batches (split-line past 8192 rows), the rest by the one-thread kernel."""
+    rng = np.random.default_rng(11)
+    lengths = np.array([150000, 40000, 9001] + list(rng.integers(4097, 4300, 150)) +
+                       list(rng.integers(1, 3000, 300)), dtype=np.int64)
+    rng.shuffle(lengths)
+    off, arrs = ragged_inputs(lengths, dtype, seed=4)
+    u = rng.uniform(-1, 1, int(off[-1])).astype(dtype) if inc else None
+    (got,), plan = run(off, arrs, dtype, inc, u)
+    desc = plan.describe()
+    assert "solo:3" in desc and "long(>4096):150" in desc, desc
+    exp_d = arrs[3].copy()
+    exp_u = u.copy() if inc else None
+    assert oracle.ragged(arrs[0], arrs[1], arrs[2], exp_d, exp_u, off, inc=inc) == 0
+    check_close(exp_u if inc else exp_d, got, dtype, f"ragged [{desc}]")
 
 
 def test_ragged_repeated_solves_one_plan():

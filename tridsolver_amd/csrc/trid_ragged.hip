@@ -17,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/trid_hip.h"
 #include "../../include/trid_hip_ext.h"
 #include "../../include/trid_ragged.h"
 #include "trid_part.hpp"
@@ -26,6 +27,13 @@ namespace tridhip {
 
 constexpr int kRagThreads = 512;
 constexpr int kRagMaxLen = 4096;  // longest system of the persistent kernel
+// cost model of the systems past kRagMaxLen rows, measured on MI355X (FP64):
+// one strided one-system solve takes ~kSoloUs (27 us for 9000 rows, 42 us
+// for 2^20 rows: launches and latency); the one-thread kernel ~kThreadRowUs
+// per row of its longest system (0.4-0.8 us: a dependent chain of global
+// accesses)
+constexpr double kSoloUs = 25.0;
+constexpr double kThreadRowUs = 0.5;
 
 // Chunk lengths of the persistent kernel.  A bin is (cps, R): cps a power of
 // two (1..512) and R the shortest of 5..8 rows with cps * R >= L, so that at
@@ -290,6 +298,9 @@ struct tridRaggedPlan_st {
   long long *long_ws_off = nullptr;
   int nlong = 0;
   long long long_ws = 0;  // elements
+  // the longest systems, each solved as a one-system strided batch (split-line
+  // solve past 8192 rows): (offset, length)
+  std::vector<std::pair<long long, int>> solo;
   size_t lds = 0;
   std::string text;
 };
@@ -329,6 +340,22 @@ tridStatus_t ragged_solve(tridRaggedPlan_st *p, const T *a, const T *b, const T 
     hipLaunchKernelGGL(k_ragged_long<T>, dim3((p->nlong + 63) / 64), dim3(64), 0, st, g,
                        p->long_ids, p->long_ws_off, p->nlong, ws);
     tridStatus_t s = check_hip(hipGetLastError(), "ragged long-system launch");
+    if (s) return s;
+  }
+  for (const auto &sl : p->solo) {
+    int dims[1] = {sl.second}, opts[1] = {0};
+    const long long o = sl.first;
+    tridStatus_t s;
+    if constexpr (sizeof(T) == 8)
+      s = inc ? tridDmtsvStridedBatchInc(a + o, b + o, c + o, d + o, u + o, 1, 0, dims, dims,
+                                         opts, 0)
+              : tridDmtsvStridedBatch(a + o, b + o, c + o, d + o, nullptr, 1, 0, dims, dims,
+                                      opts, 0);
+    else
+      s = inc ? tridSmtsvStridedBatchInc(a + o, b + o, c + o, d + o, u + o, 1, 0, dims, dims,
+                                         opts, 0)
+              : tridSmtsvStridedBatch(a + o, b + o, c + o, d + o, nullptr, 1, 0, dims, dims,
+                                      opts, 0);
     if (s) return s;
   }
   if (sync) return check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
@@ -379,12 +406,42 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
   };
   constexpr int nR = kRagRmax - kRagRmin + 1;
   std::vector<std::vector<int>> bins(16 * nR);
+  // systems past kRagMaxLen rows: the k longest are solved one by one as
+  // strided batches of one system (a few launches each, bandwidth-bound),
+  // the rest together by the one-thread-per-system kernel (time set by its
+  // longest system); k minimises k * kSoloUs + L_(k+1) * kThreadRowUs
+  std::vector<std::pair<long long, long long>> over;  // (length, system)
+  for (long long k = 0; k < nsys; k++) {
+    const long long L = off[k + 1] - off[k];
+    if (L > kRagMaxLen) over.push_back({L, k});
+  }
+  std::sort(over.begin(), over.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+  size_t nsolo = 0;
+  {
+    double best = 1e300;
+    for (size_t k = 0; k <= over.size(); k++) {
+      const double t = k * kSoloUs + (k < over.size() ? over[k].first * kThreadRowUs : 0.0);
+      if (t < best) {
+        best = t;
+        nsolo = k;
+      }
+    }
+  }
+  std::vector<char> is_solo(nsolo ? nsys : 0, 0);
+  for (size_t k = 0; k < nsolo; k++) {
+    if (over[k].first > INT32_MAX) {
+      free_plan(p);
+      return fail(TRID_STATUS_INVALID_VALUE, "system longer than 2^31 - 1 rows");
+    }
+    is_solo[over[k].second] = 1;
+    p->solo.push_back({off[over[k].second], (int)over[k].first});
+  }
   std::vector<int> longs;
   std::vector<long long> long_off;
   long long lws = 0;
   for (long long k = 0; k < nsys; k++) {
     const long long L = off[k + 1] - off[k];
-    if (L == 0) continue;
+    if (L == 0 || (nsolo && is_solo[k])) continue;
     if (L > kRagMaxLen) {
       longs.push_back((int)k);
       long_off.push_back(lws);
@@ -417,7 +474,8 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
       snprintf(buf, sizeof buf, "R%d/cps%d:%zu ", R, 1 << cl, v.size());
       desc += buf;
     }
-  if (!longs.empty()) desc += "long(>4096):" + std::to_string(longs.size());
+  if (!longs.empty()) desc += "long(>4096):" + std::to_string(longs.size()) + " ";
+  if (!p->solo.empty()) desc += "solo:" + std::to_string(p->solo.size());
   p->text = desc;
   p->nitems = (int)items.size();
   p->nlong = (int)longs.size();
