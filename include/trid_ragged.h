@@ -20,8 +20,10 @@
  * LDS) and builds a work list ordered from the most to the least expensive
  * item.  Each solve is one
  * launch of a persistent kernel whose work-groups pull items from an atomic
- * counter (reset by the kernel itself).  Systems longer than 4096 rows are
- * solved after it: the longest of them one by one, each as a strided batch
+ * counter (reset by the kernel itself).  Systems of 4097..8192 rows go to a
+ * second launch of the same kernel (512 chunks of 9..16 rows).  Systems
+ * longer than 8192 rows are solved after it: the longest of them one by
+ * one, each as a strided batch
  * of one system (tridXmtsvStridedBatch, split-line solve past 8192 rows),
  * the rest by a one-thread-per-system launch; the plan picks the split by a
  * cost model (a one-system solve ~25 us, the one-thread launch ~0.5 us per

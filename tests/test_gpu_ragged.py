@@ -4,8 +4,9 @@ Each system k of [offsets[k], offsets[k+1]) must equal the reference's 1-D
 solve of that system (oracle_ragged restates trid_scalarD per system),
 within 1e-12 (FP64) / 1e-5 (FP32) normwise, plus the reference tests'
 elementwise tolerance.  Covers every length bin (1..8, wave-level 2..64
-chunks, multi-wave up to 4096, and beyond: per-system strided solves for the
-longest systems, the one-thread kernel for the rest), empty systems,
+chunks, multi-wave up to 4096, 512 chunks of 9..16 rows up to 8192, and
+beyond: per-system strided solves for the longest systems, the one-thread
+kernel for the rest), empty systems,
 Inc, repeated solves with one plan (the persistent kernel's self-reset
 counter) and the benchmark configuration (200,000 systems, lengths
 log-uniform in [8, 4096], seed 0).
@@ -79,14 +80,17 @@ def test_ragged_every_bin(dtype, inc):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_ragged_every_chunk_length(dtype):
-    """Every bin (chunks per system 1..512, chunk length 5..8 rows) at both
-    of its edges: cps*(R-1)+1 and cps*R rows."""
+    """Every bin (chunks per system 1..512, chunk length 5..8 rows; 512
+    chunks of 9..16 rows past 4096 rows) at both of its edges: cps*(R-1)+1
+    and cps*R rows."""
     lengths = []
     for cl in range(10):
         cps = 1 << cl
         for R in range(5, 9):
             lengths += [cps * (R - 1) + 1, cps * R]
-    lengths = np.array([L for L in lengths if L <= 4096] * 2, dtype=np.int64)
+    lengths = [L for L in lengths if L <= 4096]
+    lengths += [L for R in range(9, 17) for L in (512 * (R - 1) + 1, 512 * R)]
+    lengths = np.array(lengths * 2, dtype=np.int64)
     np.random.default_rng(9).shuffle(lengths)
     off, arrs = ragged_inputs(lengths, dtype, seed=3)
     (got,), plan = run(off, arrs, dtype)
@@ -95,22 +99,26 @@ def test_ragged_every_chunk_length(dtype):
     check_close(exp, got, dtype, f"ragged [{plan.describe()}]")
     for R in range(5, 9):
         assert f"R{R}/" in plan.describe()
+    for R in range(9, 17):
+        assert f"R{R}/cps512:4" in plan.describe(), plan.describe()
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("inc", [False, True])
 def test_ragged_long_systems_both_routes(dtype, inc):
-    """Systems past 4096 rows: the longest solved one by one as strided
-    batches (split-line past 8192 rows), the rest by the one-thread kernel."""
+    """Systems of 4097..8192 rows in the second persistent launch; past 8192
+    rows the longest solved one by one as strided batches (split-line), the
+    rest by the one-thread kernel."""
     rng = np.random.default_rng(11)
-    lengths = np.array([150000, 40000, 9001] + list(rng.integers(4097, 4300, 150)) +
-                       list(rng.integers(1, 3000, 300)), dtype=np.int64)
+    lengths = np.array([150000, 40000, 20000] + list(rng.integers(8193, 8300, 300)) +
+                       list(rng.integers(4097, 8193, 20)) + list(rng.integers(1, 3000, 300)),
+                       dtype=np.int64)
     rng.shuffle(lengths)
     off, arrs = ragged_inputs(lengths, dtype, seed=4)
     u = rng.uniform(-1, 1, int(off[-1])).astype(dtype) if inc else None
     (got,), plan = run(off, arrs, dtype, inc, u)
     desc = plan.describe()
-    assert "solo:3" in desc and "long(>4096):150" in desc, desc
+    assert "solo:3" in desc and "long(>8192):300" in desc and "cps512" in desc, desc
     exp_d = arrs[3].copy()
     exp_u = u.copy() if inc else None
     assert oracle.ragged(arrs[0], arrs[1], arrs[2], exp_d, exp_u, off, inc=inc) == 0

@@ -27,7 +27,12 @@ namespace tridhip {
 
 constexpr int kRagThreads = 512;
 constexpr int kRagMaxLen = 4096;  // longest system of the persistent kernel
-// cost model of the systems past kRagMaxLen rows, measured on MI355X (FP64):
+// systems of kRagMaxLen+1 .. kRagLongLen rows: a second launch of the
+// persistent kernel with 512 chunks of 9..16 rows (more registers per lane,
+// so it does not share the first launch's instantiation)
+constexpr int kRagLongLen = 8192;
+constexpr int kRagR2min = 9, kRagR2max = 16;
+// cost model of the systems past kRagLongLen rows, measured on MI355X (FP64):
 // one strided one-system solve takes ~kSoloUs (27 us for 9000 rows, 42 us
 // for 2^20 rows: launches and latency); the one-thread kernel ~kThreadRowUs
 // per row of its longest system (0.4-0.8 us: a dependent chain of global
@@ -212,9 +217,21 @@ __device__ __forceinline__ void rag_item(const RagArgs<T> &g, const RagItem &it,
   rag_store<T, R>(dst, len, reg, kg, G, g.inc != 0);
 }
 
+// rag_item for the item's chunk length, one of RLO..RHI
+template <typename T, int R, int RHI>
+__device__ __forceinline__ void rag_dispatch(const RagArgs<T> &g, const RagItem &it, T *smem) {
+  if constexpr (R == RHI) {
+    rag_item<T, R>(g, it, smem);
+  } else {
+    if (it.rows == R) rag_item<T, R>(g, it, smem);
+    else rag_dispatch<T, R + 1, RHI>(g, it, smem);
+  }
+}
+
 // Persistent work-groups take items from an atomic counter; the next
-// item's index is fetched while the current item is processed.
-template <typename T>
+// item's index is fetched while the current item is processed.  RLO..RHI:
+// the chunk lengths of the launch's items.
+template <typename T, int RLO, int RHI>
 __global__ void __launch_bounds__(kRagThreads) k_ragged(RagArgs<T> g) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *smem = reinterpret_cast<T *>(smem_raw);
@@ -226,12 +243,7 @@ __global__ void __launch_bounds__(kRagThreads) k_ragged(RagArgs<T> g) {
     __syncthreads();  // every lane has read s_next
     if (threadIdx.x == 0) s_next = (int)atomicAdd(g.counter, 1u);
     const RagItem it = g.items[i];
-    switch (it.rows) {
-      case 5: rag_item<T, 5>(g, it, smem); break;
-      case 6: rag_item<T, 6>(g, it, smem); break;
-      case 7: rag_item<T, 7>(g, it, smem); break;
-      default: rag_item<T, 8>(g, it, smem); break;
-    }
+    rag_dispatch<T, RLO, RHI>(g, it, smem);
     __syncthreads();  // s_next written; the next item reuses the staging region
     i = s_next;
   }
@@ -294,6 +306,12 @@ struct tridRaggedPlan_st {
   int nitems = 0;
   unsigned *counter = nullptr;
   int grid = 0;
+  // second launch: systems of kRagMaxLen+1 .. kRagLongLen rows
+  int *sys2 = nullptr;
+  RagItem *items2 = nullptr;
+  int nitems2 = 0;
+  int grid2 = 0;
+  size_t lds2 = 0;
   int *long_ids = nullptr;
   long long *long_ws_off = nullptr;
   int nlong = 0;
@@ -311,6 +329,8 @@ void free_plan(tridRaggedPlan_st *p) {
   if (!p) return;
   (void)hipFree(p->sys);
   (void)hipFree(p->items);
+  (void)hipFree(p->sys2);
+  (void)hipFree(p->items2);
   (void)hipFree(p->counter);
   (void)hipFree(p->long_ids);
   (void)hipFree(p->long_ws_off);
@@ -330,7 +350,20 @@ tridStatus_t ragged_solve(tridRaggedPlan_st *p, const T *a, const T *b, const T 
   hipStream_t st = current_stream();
   RagArgs<T> g{a, b, c, d, u, inc ? 1 : 0, p->offsets, p->sys, p->items, p->nitems, p->counter};
   if (p->nitems > 0) {
-    hipLaunchKernelGGL(k_ragged<T>, dim3(p->grid), dim3(kRagThreads), p->lds, st, g);
+    hipLaunchKernelGGL((k_ragged<T, kRagRmin, kRagRmax>), dim3(p->grid), dim3(kRagThreads),
+                       p->lds, st, g);
+    tridStatus_t s = check_hip(hipGetLastError(), "ragged kernel launch");
+    if (s) return s;
+  }
+  if (p->nitems2 > 0) {
+    // stream-ordered after the first launch, whose last work-group has reset
+    // the counter
+    RagArgs<T> g2 = g;
+    g2.sys = p->sys2;
+    g2.items = p->items2;
+    g2.nitems = p->nitems2;
+    hipLaunchKernelGGL((k_ragged<T, kRagR2min, kRagR2max>), dim3(p->grid2), dim3(kRagThreads),
+                       p->lds2, st, g2);
     tridStatus_t s = check_hip(hipGetLastError(), "ragged kernel launch");
     if (s) return s;
   }
@@ -413,7 +446,7 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
   std::vector<std::pair<long long, long long>> over;  // (length, system)
   for (long long k = 0; k < nsys; k++) {
     const long long L = off[k + 1] - off[k];
-    if (L > kRagMaxLen) over.push_back({L, k});
+    if (L > kRagLongLen) over.push_back({L, k});
   }
   std::sort(over.begin(), over.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
   size_t nsolo = 0;
@@ -439,13 +472,20 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
   std::vector<int> longs;
   std::vector<long long> long_off;
   long long lws = 0;
+  constexpr int nR2 = kRagR2max - kRagR2min + 1;
+  std::vector<std::vector<int>> bins2(nR2);  // 512 chunks of R = 9..16 rows
   for (long long k = 0; k < nsys; k++) {
     const long long L = off[k + 1] - off[k];
     if (L == 0 || (nsolo && is_solo[k])) continue;
-    if (L > kRagMaxLen) {
+    if (L > kRagLongLen) {
       longs.push_back((int)k);
       long_off.push_back(lws);
       lws += 2 * L;
+      continue;
+    }
+    if (L > kRagMaxLen) {
+      const int R = (int)((L + kRagThreads - 1) / kRagThreads);
+      bins2[R - kRagR2min].push_back((int)k);
       continue;
     }
     int R = 0;
@@ -474,7 +514,25 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
       snprintf(buf, sizeof buf, "R%d/cps%d:%zu ", R, 1 << cl, v.size());
       desc += buf;
     }
-  if (!longs.empty()) desc += "long(>4096):" + std::to_string(longs.size()) + " ";
+  std::vector<int> sys2;
+  std::vector<RagItem> items2;
+  for (int R = kRagR2max; R >= kRagR2min; R--) {
+    const std::vector<int> &v = bins2[R - kRagR2min];
+    if (v.empty()) continue;
+    for (size_t i = 0; i < v.size(); i++) {
+      RagItem it;
+      it.cps_log2 = 9;
+      it.first = (int)(sys2.size() + i);
+      it.count = 1;
+      it.rows = R;
+      items2.push_back(it);
+    }
+    sys2.insert(sys2.end(), v.begin(), v.end());
+    char buf[96];
+    snprintf(buf, sizeof buf, "R%d/cps512:%zu ", R, v.size());
+    desc += buf;
+  }
+  if (!longs.empty()) desc += "long(>8192):" + std::to_string(longs.size()) + " ";
   if (!p->solo.empty()) desc += "solo:" + std::to_string(p->solo.size());
   p->text = desc;
   p->nitems = (int)items.size();
@@ -484,6 +542,8 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
   // span-group regions (at most 512 groups of G*R + 2E elements, G >= 1);
   // the LDS reduced solve reuses them (6 values per thread)
   p->lds = (size_t)kRagThreads * (kRagRmax + 2 * (16 / ts)) * ts;
+  p->lds2 = (size_t)kRagThreads * (kRagR2max + 2 * (16 / ts)) * ts;
+  p->nitems2 = (int)items2.size();
   auto up = [&](void **dst, const void *src, size_t bytes) -> tridStatus_t {
     if (bytes == 0) return TRID_STATUS_SUCCESS;
     tridStatus_t r = check_hip(hipMalloc(dst, bytes), "hipMalloc");
@@ -492,6 +552,8 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
   };
   s = up((void **)&p->sys, sys.data(), sys.size() * sizeof(int));
   if (!s) s = up((void **)&p->items, items.data(), items.size() * sizeof(RagItem));
+  if (!s) s = up((void **)&p->sys2, sys2.data(), sys2.size() * sizeof(int));
+  if (!s) s = up((void **)&p->items2, items2.data(), items2.size() * sizeof(RagItem));
   if (!s) s = up((void **)&p->long_ids, longs.data(), longs.size() * sizeof(int));
   if (!s) s = up((void **)&p->long_ws_off, long_off.data(), long_off.size() * sizeof(long long));
   if (!s) {
@@ -506,15 +568,22 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
   int dev = 0, cus = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  int per_cu = 0;
-  if (p->dbl)
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ragged<double>, kRagThreads,
-                                                       p->lds);
-  else
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ragged<float>, kRagThreads,
-                                                       p->lds);
+  int per_cu = 0, per_cu2 = 0;
+  if (p->dbl) {
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, k_ragged<double, kRagRmin, kRagRmax>, kRagThreads, p->lds);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu2, k_ragged<double, kRagR2min, kRagR2max>, kRagThreads, p->lds2);
+  } else {
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, k_ragged<float, kRagRmin, kRagRmax>, kRagThreads, p->lds);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu2, k_ragged<float, kRagR2min, kRagR2max>, kRagThreads, p->lds2);
+  }
   per_cu = std::max(1, per_cu);
+  per_cu2 = std::max(1, per_cu2);
   p->grid = std::max(1, std::min(p->nitems, cus * per_cu));
+  p->grid2 = std::max(1, std::min(p->nitems2, cus * per_cu2));
   *plan = p;
   return TRID_STATUS_SUCCESS;
 }
