@@ -36,7 +36,11 @@ EXTERN_C tridStatus_t tridAdiPlanCreate(tridAdiPlan_t *plan, int nx, int ny,
                                         int mode);
 EXTERN_C tridStatus_t tridAdiPlanDestroy(tridAdiPlan_t plan);
 
-/* Enqueues nsteps time steps; sync = 1 waits for them to finish. */
+/* Enqueues nsteps time steps; sync = 1 waits for them to finish.  From the
+ * second consecutive call with the same arrays and nsteps >= 2 on, the steps
+ * run as launches of a HIP graph of min(nsteps, 64) steps, captured at that
+ * call (the plan keeps one such graph; the kernels are those of
+ * tridAdiStage, so results are bitwise the same). */
 EXTERN_C tridStatus_t tridAdiStep(tridAdiPlan_t plan, void *u, void *du,
                                   int nsteps, int sync);
 

@@ -154,6 +154,48 @@ def test_adi_every_sweep_layout(shape, dt):
     print(f"{shape} {dt.__name__}: {ran} layouts")
 
 
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_adi_step_graph_matches_single_steps(dt):
+    """tridAdiStep with several steps launches a cached graph of up to 64
+    steps from the second call with the same arrays and count on: bitwise
+    equal to the same number of one-step calls (kernel by kernel) on the
+    first call, the capture, a replay, other arrays, another count, past 64
+    steps (64-step graph + remainder) and after re-pinning a sweep."""
+    import torch
+    nx, ny, nz = 40, 24, 16
+    u0 = torch.from_numpy(rand_init(nx, ny, nz, dt, seed=12)).to("cuda")
+    plan = AdiPlan(nx, ny, nz, torch_dt(dt), 1.0, 1)
+
+    def single(n):
+        u = u0.clone()
+        du = torch.empty_like(u)
+        for _ in range(n):
+            plan.step(u, du, 1)
+        return u.cpu().numpy()
+
+    u_a, du_a = u0.clone(), torch.empty_like(u0)
+    u_b, du_b = u0.clone(), torch.empty_like(u0)
+    exp5 = single(5)
+
+    def three_calls(u, du, n, exp):
+        for _ in range(3):                       # eager, capture, replay
+            u.copy_(u0)
+            plan.step(u, du, n)
+            np.testing.assert_array_equal(u.cpu().numpy(), exp)
+
+    three_calls(u_a, du_a, 5, exp5)
+    three_calls(u_b, du_b, 5, exp5)                    # other arrays
+    three_calls(u_b, du_b, 3, single(3))               # other count
+    three_calls(u_b, du_b, 70, single(70))             # 64-step graph + 6
+    plan.step(u_b, du_b, 5)
+    plan.step(u_b, du_b, 5)                      # a graph is held
+    plan.set_sweep(1, 16, 0, 1)                  # drops it
+    three_calls(u_b, du_b, 5, single(5))
+    exp = u0.cpu().numpy().ravel().copy()
+    assert oracle.adi(exp, nx, ny, nz, 5, 1.0, 1) == 0
+    assert oracle.rel_err(exp, u_b.cpu().numpy().ravel()) <= tol(dt)
+
+
 def test_adi_stages_vs_oracle():
     """RHS bit-exact; each sweep against the oracle's strided solve."""
     import torch

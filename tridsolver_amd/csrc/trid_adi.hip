@@ -218,9 +218,14 @@ struct tridAdiPlan_st {
   int cc_pair[3] = {0, 0, 0};  // two adjacent systems per lane (16-byte rows)
   void *cc_F[3] = {nullptr, nullptr, nullptr};
   bool tune_pending = false;
-  hipGraph_t graph = nullptr;
+  hipGraph_t graph = nullptr;  // tridAdiGraphCapture's graph
   hipGraphExec_t exec = nullptr;
   hipStream_t cap = nullptr;
+  // tridAdiStep's graph of step_n steps on (step_u, step_du)
+  hipGraph_t step_graph = nullptr;
+  hipGraphExec_t step_exec = nullptr;
+  void *step_u = nullptr, *step_du = nullptr;
+  int step_n = 0;
   std::string text;
 };
 
@@ -739,10 +744,63 @@ EXTERN_C tridStatus_t tridAdiPlanCreate(tridAdiPlan_t *plan, int nx, int ny,
   return TRID_STATUS_SUCCESS;
 }
 
+namespace {
+
+// longest step graph tridAdiStep captures (longer runs replay it)
+constexpr int kStepGraphMax = 64;
+
+// Drops the plan's graphs (both hold the factor tables and the kernel
+// choices of the time they were captured), after any launch of them ended.
+void drop_graphs(tridAdiPlan_st *p, bool explicit_graph) {
+  p->step_u = p->step_du = nullptr;
+  p->step_n = 0;
+  if (!p->step_exec && !(explicit_graph && p->exec)) return;
+  (void)hipDeviceSynchronize();
+  if (p->step_exec) (void)hipGraphExecDestroy(p->step_exec);
+  if (p->step_graph) (void)hipGraphDestroy(p->step_graph);
+  p->step_exec = nullptr;
+  p->step_graph = nullptr;
+  if (explicit_graph) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->exec = nullptr;
+    p->graph = nullptr;
+  }
+}
+
+// Captures nsteps steps on (u, du) into a graph on the plan's capture stream.
+tridStatus_t capture_steps(tridAdiPlan_st *p, void *u, void *du, int nsteps, hipGraph_t *graph,
+                           hipGraphExec_t *exec) {
+  if (!p->cap) {
+    tridStatus_t s = check_hip(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking),
+                               "hipStreamCreate");
+    if (s) return s;
+  }
+  tridStatus_t s = check_hip(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal),
+                             "hipStreamBeginCapture");
+  if (s) return s;
+  tridStatus_t s2 = steps_any(p, u, du, nsteps, p->cap);
+  hipGraph_t g = nullptr;
+  s = check_hip(hipStreamEndCapture(p->cap, &g), "hipStreamEndCapture");
+  if (s2) {
+    if (g) (void)hipGraphDestroy(g);
+    return s2;
+  }
+  if (s) return s;
+  s = check_hip(hipGraphInstantiate(exec, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+  if (s) {
+    (void)hipGraphDestroy(g);
+    return s;
+  }
+  *graph = g;
+  return TRID_STATUS_SUCCESS;
+}
+
+}  // namespace
+
 EXTERN_C tridStatus_t tridAdiPlanDestroy(tridAdiPlan_t p) {
   if (!p) return TRID_STATUS_SUCCESS;
-  if (p->exec) (void)hipGraphExecDestroy(p->exec);
-  if (p->graph) (void)hipGraphDestroy(p->graph);
+  drop_graphs(p, true);
   if (p->cap) (void)hipStreamDestroy(p->cap);
   for (void *F : p->cc_F)
     if (F) (void)hipFree(F);
@@ -758,7 +816,35 @@ EXTERN_C tridStatus_t tridAdiStep(tridAdiPlan_t p, void *u, void *du, int nsteps
   if (nsteps < 0) return fail(TRID_STATUS_INVALID_VALUE, "nsteps < 0");
   ensure_tuned(p, u, du);
   hipStream_t st = current_stream();
-  s = steps_any(p, u, du, nsteps, st);
+  // several steps: from the second consecutive call with these arrays and
+  // that count on, launches of a graph of up to kStepGraphMax steps, which
+  // removes the launch gaps between the kernels (a one-off call does not
+  // pay for a capture); the first call, the remainder and calls on a stream
+  // that is itself capturing are launched kernel by kernel
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  const hipError_t qe = hipStreamIsCapturing(st, &cs);
+  if (qe != hipSuccess) (void)hipGetLastError();
+  const bool capturing = qe != hipSuccess || cs != hipStreamCaptureStatusNone;
+  int rest = nsteps;
+  if (nsteps >= 2 && !capturing) {
+    const int chunk = std::min(nsteps, kStepGraphMax);
+    if (p->step_u == u && p->step_du == du && p->step_n == chunk) {
+      if (!p->step_exec) {
+        s = capture_steps(p, u, du, chunk, &p->step_graph, &p->step_exec);
+        if (s) return s;
+      }
+      for (; rest >= chunk; rest -= chunk) {
+        s = check_hip(hipGraphLaunch(p->step_exec, st), "hipGraphLaunch");
+        if (s) return s;
+      }
+    } else {
+      drop_graphs(p, false);
+      p->step_u = u;
+      p->step_du = du;
+      p->step_n = chunk;
+    }
+  }
+  if (rest > 0) s = steps_any(p, u, du, rest, st);
   if (s) return s;
   if (sync) return check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
   return TRID_STATUS_SUCCESS;
@@ -782,12 +868,8 @@ EXTERN_C tridStatus_t tridAdiGraphCapture(tridAdiPlan_t p, void *u, void *du,
   if (s) return s;
   if (nsteps < 1) return fail(TRID_STATUS_INVALID_VALUE, "nsteps must be >= 1");
   ensure_tuned(p, u, du);
-  if (!p->cap) {
-    s = check_hip(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking),
-                  "hipStreamCreate");
-    if (s) return s;
-  }
   if (p->exec) {
+    (void)hipDeviceSynchronize();  // a launch of the old graph may still run
     (void)hipGraphExecDestroy(p->exec);
     p->exec = nullptr;
   }
@@ -795,20 +877,7 @@ EXTERN_C tridStatus_t tridAdiGraphCapture(tridAdiPlan_t p, void *u, void *du,
     (void)hipGraphDestroy(p->graph);
     p->graph = nullptr;
   }
-  s = check_hip(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal),
-                "hipStreamBeginCapture");
-  if (s) return s;
-  tridStatus_t s2 = steps_any(p, u, du, nsteps, p->cap);
-  hipGraph_t g = nullptr;
-  s = check_hip(hipStreamEndCapture(p->cap, &g), "hipStreamEndCapture");
-  if (s2) {
-    if (g) (void)hipGraphDestroy(g);
-    return s2;
-  }
-  if (s) return s;
-  p->graph = g;
-  return check_hip(hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0),
-                   "hipGraphInstantiate");
+  return capture_steps(p, u, du, nsteps, &p->graph, &p->exec);
 }
 
 EXTERN_C tridStatus_t tridAdiGraphLaunch(tridAdiPlan_t p, int sync) {
@@ -845,7 +914,9 @@ EXTERN_C tridStatus_t tridAdiPlanSetSweep(tridAdiPlan_t p, int dir, int rows, in
   } else if (pair || xcd) {
     return fail(TRID_STATUS_INVALID_VALUE, "pair and xcd apply to rows 8, 16 and 32");
   }
-  // hipFree waits for queued kernels that may still read the old factors
+  // graphs hold the old factor table and kernel choice; hipFree waits for
+  // queued kernels that may still read the old factors
+  drop_graphs(p, true);
   if (p->cc_F[dir]) (void)hipFree(p->cc_F[dir]);
   p->cc_F[dir] = F;
   p->cc_R[dir] = rows;
