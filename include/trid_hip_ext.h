@@ -43,7 +43,8 @@ EXTERN_C const char *tridGetLastConfig(void);
  * rows * cps rows that are reduced, coupled through a reduced system of two
  * rows per segment and solved as one batch; automatic for systems longer
  * than 8192 rows).  rows is the chunk length R of kinds 1, 3 and 4 (4, 8 or
- * 16; 0 = automatic), cps the chunks per system (kind 4: per segment; power
+ * 16; also 5, 6, 7 and 12 for kinds 1 and 3 along solvedim 0 with 16-byte
+ * aligned arrays, the general staged layout; 0 = automatic), cps the chunks per system (kind 4: per segment; power
  * of two; 0 = automatic), spb the systems (kind 4: segments) per work-group
  * (0 = automatic). */
 EXTERN_C tridStatus_t tridForceConfig(int kind, int rows, int cps, int spb);

@@ -189,6 +189,35 @@ def test_staged_multiwave_systems(R, dtype):
         ta.force_config(0, 0, 0, 0)
 
 
+@pytest.mark.parametrize("R", [5, 6, 7, 12])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_general_staging_fill_rows(R, dtype):
+    """Chunk lengths 5, 6, 7 and 12 (general staged layout only): several
+    lengths, dense or padded, Inc, partial waves; solvedim 1 refuses them."""
+    ta.force_config(1, R, 0, 0)
+    try:
+        cases = [([300, 21], None, False), ([77, 9], None, True), ([3000, 3], None, False),
+                 ([5, 41], None, True), ([1500, 5], [1504, 5], False), ([6000, 2], None, True)]
+        for dims, pads, inc in cases:
+            if (dims[0] + R - 1) // R > 512:  # more than 512 chunks
+                continue
+            run_case(dims, 0, dtype, inc, pads=pads, seed=dims[0] + R)
+            assert f"R{R}," in ta.last_config() and "gstaged" in ta.last_config(), \
+                ta.last_config()
+        with pytest.raises(AssertionError, match="general staged"):
+            run_case([30, 40], 1, dtype, False)
+    finally:
+        ta.force_config(0, 0, 0, 0)
+
+
+@pytest.mark.parametrize("N,R", [(300, 5), (200, 7), (3000, 6), (1500, 6), (6000, 12)])
+def test_default_fill_rows(N, R):
+    """Without tuning, FP64 staged solves take the chunk length that idles the
+    fewest rows."""
+    run_case([N, 7], 0, np.float64, False, seed=N)
+    assert f"R{R}," in ta.last_config(), ta.last_config()
+
+
 @pytest.mark.parametrize("R", [4, 8, 16])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_general_staging_any_length(R, dtype):

@@ -131,6 +131,11 @@ hipError_t launch_part(const PartCfg &cfg, const Coef &coef, T *d, T *u,
       TRID_CASE(0, 4, 4)
       TRID_CASE(0, 8, 4)
       TRID_CASE(0, 16, 4)
+      // chunk lengths that fill a power-of-two chunk count better
+      TRID_CASE(0, 5, 4)
+      TRID_CASE(0, 6, 4)
+      TRID_CASE(0, 7, 4)
+      TRID_CASE(0, 12, 4)
     }
     TRID_CASE(1, 4, 0)
     TRID_CASE(1, 8, 0)
@@ -184,6 +189,11 @@ hipError_t launch_seg(const PartCfg &cfg, const CoefArrays<T> &coef, T *d, T *u,
   return hipErrorInvalidValue;
 }
 
+// Chunk lengths of the general staged layout (VEC 4) besides 4, 8 and 16:
+// with a power-of-two chunk count they leave fewer rows idle for lengths
+// between powers of two (300 rows: 60 chunks of 5 in 64, against 38 of 8).
+inline bool is_fill_rows(int R) { return R == 5 || R == 6 || R == 7 || R == 12; }
+
 // tsize: element size; FP32 16-row chunks of strided systems (ORDER 1,
 // unstaged) fit 1024-thread work-groups (79 VGPRs), FP64 ones and staged
 // layouts do not.
@@ -195,6 +205,10 @@ inline bool cfg_valid(const PartCfg &cfg, int N, size_t tsize = 8) {
     return cfg_valid(s, cfg.R << cfg.cps_log2, tsize);
   }
   if (cfg.kind != 1) return true;
+  // chunk lengths 5, 6, 7 and 12 exist for the general staged layout only
+  if (cfg.R != 4 && cfg.R != 8 && cfg.R != 16 &&
+      !(is_fill_rows(cfg.R) && cfg.vec == 4 && cfg.order == 0))
+    return false;
   const long long cps = 1LL << cfg.cps_log2, spb = 1LL << cfg.spb_log2;
   const long long thr = cps * spb;
   if (thr < kWave || thr > 1024) return false;
@@ -296,7 +310,37 @@ inline std::vector<PartCfg> candidates(int order, int N, int vec, size_t tsize,
         add_both(c);
       }
     }
+  for (int R : {5, 6, 7, 12})
+    for (int t : thr) {
+      PartCfg c = make_cfg(order, R, N, 0, t);
+      if (!gstage_ok(c, gvec, dense)) continue;
+      c.vec = 4;
+      if (cfg_valid(c, N, tsize)) add_both(c);
+    }
   return out;
+}
+
+// General staged layout: the chunk length among 4..8, 12 and 16 that leaves
+// the fewest idle rows (cps * R - N with cps the power of two >= N / R) at
+// the configuration's work-group size; ties keep c.
+inline PartCfg fill_rows(const PartCfg &c, int N, bool gvec, bool dense) {
+  const int thr = 1 << (c.cps_log2 + c.spb_log2);
+  PartCfg best = c;
+  long long waste = ((long long)c.R << c.cps_log2) - N;
+  for (int R : {4, 5, 6, 7, 8, 12, 16}) {
+    PartCfg t = make_cfg(0, R, N, 0, thr);
+    if (!gstage_ok(t, gvec, dense)) continue;
+    t.vec = 4;
+    if (!cfg_valid(t, N, 8)) continue;
+    const long long w = ((long long)R << t.cps_log2) - N;
+    if (w < waste) {
+      waste = w;
+      best = t;
+    }
+  }
+  best.xcd = c.xcd;
+  best.idx32 = c.idx32;
+  return best;
 }
 
 // Autotuner timing: `passes` interleaved rounds over the candidates; in each

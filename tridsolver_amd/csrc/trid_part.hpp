@@ -994,7 +994,8 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
   } else if (full) {
     I oa = off[0] + (I)row0 * s0, ob = off[1] + (I)row0 * s1,
       oc = off[2] + (I)row0 * s2, od = off[3] + (I)row0 * s3;
-    if (VEC) {
+    // (staged layouts never get here: VEC is 0 or 1)
+    if constexpr (VEC == 1) {
       if constexpr (Coef::kArrays) {
         load_chunk_vec<T, R>(at(coef.A, oa), a);
         load_chunk_vec<T, R>(at(coef.B, ob), b);
@@ -1184,7 +1185,7 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
   } else if (full) {
     if (INC) {
       I ou = off[4] + (I)row0 * s4;
-      if (VEC) {
+      if constexpr (VEC == 1) {
         T u[R];
         load_chunk_vec<T, R>(at(U, ou), u);
 #pragma unroll
@@ -1199,7 +1200,7 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
       }
     } else {
       I od = off[3] + (I)row0 * s3;
-      if (VEC) {
+      if constexpr (VEC == 1) {
         store_chunk_vec<T, R>(at(Dst, od), y);
       } else {
 #pragma unroll

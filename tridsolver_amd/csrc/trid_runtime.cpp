@@ -128,8 +128,9 @@ EXTERN_C const char *tridGetStatusString(tridStatus_t s) {
 
 EXTERN_C tridStatus_t tridForceConfig(int kind, int rows, int cps, int spb) {
   if (kind < 0 || kind > 4) return fail(TRID_STATUS_INVALID_VALUE, "tridForceConfig: kind");
-  if (rows != 0 && rows != 4 && rows != 8 && rows != 16)
-    return fail(TRID_STATUS_INVALID_VALUE, "tridForceConfig: rows must be 0, 4, 8 or 16");
+  if (rows != 0 && rows != 4 && rows != 5 && rows != 6 && rows != 7 && rows != 8 &&
+      rows != 12 && rows != 16)
+    return fail(TRID_STATUS_INVALID_VALUE, "tridForceConfig: rows must be 0, 4..8, 12 or 16");
   auto pow2 = [](int v) { return v == 0 || (v > 0 && (v & (v - 1)) == 0); };
   if (!pow2(cps) || !pow2(spb))
     return fail(TRID_STATUS_INVALID_VALUE, "tridForceConfig: cps/spb must be powers of two");

@@ -438,6 +438,13 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
                    order == 0 ? 256 : 512);
     if (fc.cps) cfg.cps_log2 = ilog2(fc.cps);
     if (fc.spb) cfg.spb_log2 = ilog2(fc.spb);
+    if (is_fill_rows(R)) {
+      // these chunk lengths exist in the general staged layout only
+      if (!gstage_ok(cfg, gvec, dense))
+        return fail(TRID_STATUS_INVALID_VALUE,
+                    "rows 5, 6, 7 and 12 need the general staged layout");
+      cfg.vec = 4;
+    }
     if (!cfg_valid(cfg, D.N, sizeof(T)))
       return fail(TRID_STATUS_INVALID_VALUE, "forced configuration does not fit this shape");
     if (fc.stage && stage_ok(cfg, dense, D.nsys, D.N)) cfg.vec = staged_vec(cfg);
@@ -477,6 +484,8 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
             gstage_ok(c16, gvec, dense))
           cfg = c16;
         if (gstage_ok(cfg, gvec, dense)) cfg.vec = 4;
+        // FP64: the chunk length that idles the fewest rows
+        if (cfg.vec == 4 && sizeof(T) == 8) cfg = fill_rows(cfg, D.N, gvec, dense);
       }
     }
   }
