@@ -210,12 +210,18 @@ def test_general_staging_fill_rows(R, dtype):
         ta.force_config(0, 0, 0, 0)
 
 
-@pytest.mark.parametrize("N,R", [(300, 5), (200, 7), (3000, 6), (1500, 6), (6000, 12)])
-def test_default_fill_rows(N, R):
-    """Without tuning, FP64 staged solves take the chunk length that idles the
-    fewest rows."""
-    run_case([N, 7], 0, np.float64, False, seed=N)
-    assert f"R{R}," in ta.last_config(), ta.last_config()
+@pytest.mark.parametrize("dtype,N,cfg", [
+    (np.float64, 300, "R12,cps32"), (np.float64, 200, "R7,cps32"), (np.float64, 3000, "R12,cps256"),
+    (np.float64, 3500, "R7,cps512"), (np.float64, 500, "R8,cps64"),
+    (np.float64, 600, "R12,cps64"), (np.float64, 1000, "R8,cps128"), (np.float64, 6000, "R12,cps512"),
+    (np.float64, 7000, "R16,cps512"), (np.float32, 300, "R12,cps32"), (np.float32, 200, "R4,cps64"),
+    (np.float32, 5000, "R12,cps512"), (np.float32, 8000, "R8,cps1024")])
+def test_default_chunk_length(dtype, N, cfg):
+    """Without tuning, solvedim-0 solves take 12-row (FP64 also 7-row) chunks
+    with general staging where they pad the system to fewer rows than the
+    8-row (FP64) or 4-row (FP32) chunks."""
+    run_case([N, 7], 0, dtype, False, seed=N)
+    assert cfg in ta.last_config(), ta.last_config()
 
 
 @pytest.mark.parametrize("R", [4, 8, 16])

@@ -320,26 +320,27 @@ inline std::vector<PartCfg> candidates(int order, int N, int vec, size_t tsize,
   return out;
 }
 
-// General staged layout: the chunk length among 4..8, 12 and 16 that leaves
-// the fewest idle rows (cps * R - N with cps the power of two >= N / R) at
-// the configuration's work-group size; ties keep c.
-inline PartCfg fill_rows(const PartCfg &c, int N, bool gvec, bool dense) {
-  const int thr = 1 << (c.cps_log2 + c.spb_log2);
+// Default chunk length of solvedim-0 solves that are not exactly staged:
+// the configuration c (FP64: 8 or 16 rows, general staging; FP32: 4 or 8
+// rows, 16-byte loads) or general staging with 12-row chunks (FP64 also
+// 7-row chunks), whichever pads the system to the fewest rows (cps * R, cps
+// the power of two >= N / R); ties keep c.  From tuning sweeps on MI355X
+// (N = 70..8000): 12- and 7-row chunks fill the lengths between the powers
+// of two that 8-row (or 4-row) chunks fit, and win there by 10-30 %.
+inline PartCfg pick_rows(const PartCfg &c, int N, bool gvec, bool dense, size_t tsize) {
+  if (c.kind != 1 || c.order != 0 || c.vec == 2 || c.vec == 3) return c;
   PartCfg best = c;
-  long long waste = ((long long)c.R << c.cps_log2) - N;
-  for (int R : {4, 5, 6, 7, 8, 12, 16}) {
-    PartCfg t = make_cfg(0, R, N, 0, thr);
+  for (int R : {12, 7}) {
+    if (R == 7 && tsize != 8) continue;
+    PartCfg t = make_cfg(0, R, N, 0, 256);
     if (!gstage_ok(t, gvec, dense)) continue;
     t.vec = 4;
-    if (!cfg_valid(t, N, 8)) continue;
-    const long long w = ((long long)R << t.cps_log2) - N;
-    if (w < waste) {
-      waste = w;
-      best = t;
-    }
+    if (!cfg_valid(t, N, tsize)) continue;
+    if (((long long)t.R << t.cps_log2) >= ((long long)best.R << best.cps_log2)) continue;
+    t.xcd = c.xcd;
+    t.idx32 = c.idx32;
+    best = t;
   }
-  best.xcd = c.xcd;
-  best.idx32 = c.idx32;
   return best;
 }
 

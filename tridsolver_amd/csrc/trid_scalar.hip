@@ -484,10 +484,10 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
             gstage_ok(c16, gvec, dense))
           cfg = c16;
         if (gstage_ok(cfg, gvec, dense)) cfg.vec = 4;
-        // FP64: the chunk length that idles the fewest rows
-        if (cfg.vec == 4 && sizeof(T) == 8) cfg = fill_rows(cfg, D.N, gvec, dense);
       }
     }
+    // solvedim 0 between powers of two: 12-row chunks when they pad less
+    cfg = pick_rows(cfg, D.N, gvec, dense, sizeof(T));
   }
   // systems longer than one work-group holds: split-line solve.  Strided
   // systems (ORDER 1) of 2048 rows and more take it too: a work-group that
