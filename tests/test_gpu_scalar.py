@@ -224,6 +224,20 @@ def test_default_chunk_length(dtype, N, cfg):
     assert cfg in ta.last_config(), ta.last_config()
 
 
+@pytest.mark.parametrize("dtype,N,cfg", [
+    (np.float64, 200, "R8,cps32,spb16>"), (np.float64, 300, "R16,cps32,spb16>"),
+    (np.float64, 1500, "R16,cps128,spb4,xcd>"), (np.float32, 100, "R8,cps16,spb64>"),
+    (np.float32, 200, "R8,cps32,spb32,xcd>"), (np.float32, 700, "R16,cps64,spb16,xcd>")])
+@pytest.mark.parametrize("sd", [1, 2])
+def test_default_strided_layout(dtype, N, cfg, sd):
+    """Without tuning, strided batches (solvedim >= 1) take the longest chunks
+    at the largest work-group the registers allow (most systems side by
+    side)."""
+    dims = [64, N, 3] if sd == 1 else [64, 3, N]
+    run_case(dims, sd, dtype, False, seed=N)
+    assert ta.last_config().endswith(cfg), ta.last_config()
+
+
 @pytest.mark.parametrize("R", [4, 8, 16])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_general_staging_any_length(R, dtype):

@@ -320,6 +320,26 @@ inline std::vector<PartCfg> candidates(int order, int N, int vec, size_t tsize,
   return out;
 }
 
+// Default layout of strided (solvedim >= 1) batches with coefficient arrays:
+// as many systems side by side as the register budget allows, i.e. the
+// longest chunks at the largest work-group (FP64: 16 rows at 512 threads
+// past 256 rows; FP32: 8 rows up to 256 rows, else 16, at 1024 threads),
+// with the XCD-contiguous order past 512 rows (FP32: past 128).  From tuning
+// sweeps on MI355X over [256, N, k] batches of 2^24 unknowns: at N = 300-512
+// this moves FP64 from 33 % to 50-61 % of 8 TB/s and FP32 from 31 % to
+// 53-59 %; at N = 1500 FP64 from 8 % to 36 %.  Returns kind 0 when the rule
+// does not apply (FP64 up to 256 rows keeps the generic choice).
+inline PartCfg strided_default(int N, size_t tsize) {
+  PartCfg c;
+  c.kind = 0;
+  if (tsize == 8 && N <= 256) return c;
+  const int R = (tsize == 4 && N <= 256) ? 8 : 16;
+  PartCfg t = make_cfg(1, R, N, 0, tsize == 8 ? 512 : 1024);
+  t.xcd = N > (tsize == 8 ? 512 : 128);
+  if (!cfg_valid(t, N, tsize)) return c;
+  return t;
+}
+
 // Default chunk length of solvedim-0 solves that are not exactly staged:
 // the configuration c (FP64: 8 or 16 rows, general staging; FP32: 4 or 8
 // rows, 16-byte loads) or general staging with 12-row chunks (FP64 also

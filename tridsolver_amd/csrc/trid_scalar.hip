@@ -472,6 +472,10 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
     }
   } else {
     cfg = heuristic(order, D.N, vec, sizeof(T));
+    if (order == 1 && cfg.kind == 1) {
+      const PartCfg s = strided_default(D.N, sizeof(T));
+      if (s.kind == 1) cfg = s;
+    }
     // staging pays when a chunk is more than one 16-byte vector
     if (cfg.R * sizeof(T) > 16) {
       if (stage_ok(cfg, dense, D.nsys, D.N)) {
