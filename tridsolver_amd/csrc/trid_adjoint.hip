@@ -85,6 +85,11 @@ tridStatus_t strided_adjoint(const T *a, const T *b, const T *c, const T *x, con
   const int order = solvedim == 0 ? 0 : 1;
   PartCfg cfg = heuristic(order, D.N, 0, sizeof(T));
   if (cfg.kind != 1) return fail(TRID_STATUS_INVALID_VALUE, "system too long for the adjoint kernel");
+  if (order == 1) {
+    // the strided batches' layout (most systems per work-group)
+    const PartCfg s = strided_default(D.N, sizeof(T));
+    if (s.kind == 1) cfg = s;
+  }
   cfg.vec = 0;
   cfg.idx32 = (unsigned long long)ext * sizeof(T) < (1ULL << 32) && !wide_index();
   // lambda = A^T^{-1} g, written to grad_d (g itself is not modified)
