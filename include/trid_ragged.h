@@ -14,14 +14,14 @@
  *
  * tridRaggedPlanCreate reads the offsets once (device array of nsys+1
  * int64 values, non-decreasing), bins the systems by length into chunked
- * schedules (a power-of-two number of chunks of 5..8 rows; one lane per
- * system up to 8 rows, 2..64 lanes per system with an in-wave reduced
- * solve, 128..512 lanes per system across the waves of a work-group through
- * LDS) and builds a work list ordered from the most to the least expensive
- * item.  Each solve is one
+ * schedules (a power-of-two number of chunks of 5..8 rows, FP32 9..16
+ * rows: one lane per system up to 8 rows (FP32 16), 2..64 lanes per system
+ * with an in-wave reduced solve, 128..512 lanes per system across the waves
+ * of a work-group through LDS) and builds a work list ordered from the most
+ * to the least expensive item.  Each solve is one
  * launch of a persistent kernel whose work-groups pull items from an atomic
- * counter (reset by the kernel itself).  Systems of 4097..8192 rows go to a
- * second launch of the same kernel (512 chunks of 9..16 rows).  Systems
+ * counter (reset by the kernel itself).  FP64 systems of 4097..8192 rows
+ * go to a second launch of the same kernel (512 chunks of 9..16 rows).  Systems
  * longer than 8192 rows are solved after it: the longest of them one by
  * one, each as a strided batch
  * of one system (tridXmtsvStridedBatch, split-line solve past 8192 rows),

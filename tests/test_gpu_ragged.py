@@ -80,16 +80,18 @@ def test_ragged_every_bin(dtype, inc):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_ragged_every_chunk_length(dtype):
-    """Every bin (chunks per system 1..512, chunk length 5..8 rows; 512
-    chunks of 9..16 rows past 4096 rows) at both of its edges: cps*(R-1)+1
-    and cps*R rows."""
+    """Every bin at both of its edges, cps*(R-1)+1 and cps*R rows: FP64
+    chunks of 5..8 rows (1..512 chunks per system, up to 4096 rows) and 512
+    chunks of 9..16 rows (the second launch, up to 8192 rows); FP32 chunks
+    of 9..16 rows (1..512 chunks, up to 8192 rows)."""
+    rmin, rmax = (5, 8) if dtype == np.float64 else (9, 16)
     lengths = []
     for cl in range(10):
         cps = 1 << cl
-        for R in range(5, 9):
+        for R in range(rmin, rmax + 1):
             lengths += [cps * (R - 1) + 1, cps * R]
-    lengths = [L for L in lengths if L <= 4096]
-    lengths += [L for R in range(9, 17) for L in (512 * (R - 1) + 1, 512 * R)]
+    if dtype == np.float64:
+        lengths += [L for R in range(9, 17) for L in (512 * (R - 1) + 1, 512 * R)]
     lengths = np.array(lengths * 2, dtype=np.int64)
     np.random.default_rng(9).shuffle(lengths)
     off, arrs = ragged_inputs(lengths, dtype, seed=3)
@@ -97,10 +99,11 @@ def test_ragged_every_chunk_length(dtype):
     exp = arrs[3].copy()
     assert oracle.ragged(arrs[0], arrs[1], arrs[2], exp, None, off) == 0
     check_close(exp, got, dtype, f"ragged [{plan.describe()}]")
-    for R in range(5, 9):
-        assert f"R{R}/" in plan.describe()
-    for R in range(9, 17):
+    for R in range(rmin, rmax + 1):
         assert f"R{R}/cps512:4" in plan.describe(), plan.describe()
+    if dtype == np.float64:
+        for R in range(9, 17):
+            assert f"R{R}/cps512:4" in plan.describe(), plan.describe()
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])

@@ -41,17 +41,20 @@ constexpr double kSoloUs = 25.0;
 constexpr double kThreadRowUs = 0.5;
 
 // Chunk lengths of the persistent kernel.  A bin is (cps, R): cps a power of
-// two (1..512) and R the shortest of 5..8 rows with cps * R >= L, so that at
-// least 4/5 of a system's lanes hold rows (a power-of-two chunk count with
-// one fixed R leaves a quarter idle on average).  The kernel's registers are
-// those of the longest, 8 rows (512 chunks of 8 rows cover 4096 rows).
+// two (1..512) and R the shortest of 5..8 rows (FP64) with cps * R >= L, so
+// that at least 4/5 of a system's lanes hold rows (a power-of-two chunk count
+// with one fixed R leaves a quarter idle on average).  The kernel's registers
+// are those of the longest, 8 rows (512 chunks of 8 rows cover 4096 rows).
+// FP32 rows take half the registers: its bins use 9..16 rows (kRagR2min..
+// kRagR2max), half the chunks and reduced-system levels, up to 8192 rows in
+// the first launch.
 constexpr int kRagRmin = 5, kRagRmax = 8;
 
 struct RagItem {
   int cps_log2;  // chunks per system
   int first;     // first entry of the item in the sorted system list
   int count;     // systems in this item (<= kRagThreads / cps)
-  int rows;      // chunk length R of the item's bin (5..8)
+  int rows;      // chunk length R of the item's bin (5..8, or 9..16)
 };
 
 template <typename T>
@@ -350,8 +353,13 @@ tridStatus_t ragged_solve(tridRaggedPlan_st *p, const T *a, const T *b, const T 
   hipStream_t st = current_stream();
   RagArgs<T> g{a, b, c, d, u, inc ? 1 : 0, p->offsets, p->sys, p->items, p->nitems, p->counter};
   if (p->nitems > 0) {
-    hipLaunchKernelGGL((k_ragged<T, kRagRmin, kRagRmax>), dim3(p->grid), dim3(kRagThreads),
-                       p->lds, st, g);
+    // FP64 items have chunks of 5..8 rows, FP32 ones of 9..16
+    if constexpr (sizeof(T) == 8)
+      hipLaunchKernelGGL((k_ragged<T, kRagRmin, kRagRmax>), dim3(p->grid), dim3(kRagThreads),
+                         p->lds, st, g);
+    else
+      hipLaunchKernelGGL((k_ragged<T, kRagR2min, kRagR2max>), dim3(p->grid), dim3(kRagThreads),
+                         p->lds, st, g);
     tridStatus_t s = check_hip(hipGetLastError(), "ragged kernel launch");
     if (s) return s;
   }
@@ -429,15 +437,19 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
       free_plan(p);
       return fail(TRID_STATUS_INVALID_VALUE, "offsets must be non-decreasing");
     }
-  // bins (cps = 2^cl >= L / 8, R = the shortest of 5..8 rows with cps*R >= L)
+  // bins (cps = 2^cl >= L / rmax, R = the shortest of rmin..rmax rows with
+  // cps*R >= L): FP64 chunks of 5..8 rows; FP32 (half the registers per
+  // row) chunks of 9..16 rows, half the chunks and reduced-system levels
+  const int rmin = p->dbl ? kRagRmin : kRagR2min, rmax = p->dbl ? kRagRmax : kRagR2max;
+  const long long max_len = (long long)kRagThreads * rmax;  // first launch
   auto bin_of = [&](long long L, int &R) {
-    const long long ch = (L + kRagRmax - 1) / kRagRmax;
+    const long long ch = (L + rmax - 1) / rmax;
     int cl = 0;
     while ((1LL << cl) < ch) cl++;
-    R = (int)std::max<long long>(kRagRmin, (L + (1LL << cl) - 1) >> cl);
+    R = (int)std::max<long long>(rmin, (L + (1LL << cl) - 1) >> cl);
     return cl;
   };
-  constexpr int nR = kRagRmax - kRagRmin + 1;
+  const int nR = rmax - rmin + 1;
   std::vector<std::vector<int>> bins(16 * nR);
   // systems past kRagMaxLen rows: the k longest are solved one by one as
   // strided batches of one system (a few launches each, bandwidth-bound),
@@ -483,22 +495,22 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
       lws += 2 * L;
       continue;
     }
-    if (L > kRagMaxLen) {
+    if (L > max_len) {
       const int R = (int)((L + kRagThreads - 1) / kRagThreads);
       bins2[R - kRagR2min].push_back((int)k);
       continue;
     }
     int R = 0;
     const int cl = bin_of(L, R);
-    bins[cl * nR + (R - kRagRmin)].push_back((int)k);
+    bins[cl * nR + (R - rmin)].push_back((int)k);
   }
   // items, most expensive first (more chunks per system, then longer chunks)
   std::vector<int> sys;
   std::vector<RagItem> items;
   std::string desc;
   for (int cl = 15; cl >= 0; cl--)
-    for (int R = kRagRmax; R >= kRagRmin; R--) {
-      std::vector<int> &v = bins[cl * nR + (R - kRagRmin)];
+    for (int R = rmax; R >= rmin; R--) {
+      std::vector<int> &v = bins[cl * nR + (R - rmin)];
       if (v.empty()) continue;
       const int per = std::max(1, kRagThreads >> cl);
       for (size_t i = 0; i < v.size(); i += per) {
@@ -541,7 +553,7 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
   const size_t ts = p->dbl ? 8 : 4;
   // span-group regions (at most 512 groups of G*R + 2E elements, G >= 1);
   // the LDS reduced solve reuses them (6 values per thread)
-  p->lds = (size_t)kRagThreads * (kRagRmax + 2 * (16 / ts)) * ts;
+  p->lds = (size_t)kRagThreads * (rmax + 2 * (16 / ts)) * ts;
   p->lds2 = (size_t)kRagThreads * (kRagR2max + 2 * (16 / ts)) * ts;
   p->nitems2 = (int)items2.size();
   auto up = [&](void **dst, const void *src, size_t bytes) -> tridStatus_t {
@@ -576,7 +588,7 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
         &per_cu2, k_ragged<double, kRagR2min, kRagR2max>, kRagThreads, p->lds2);
   } else {
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, k_ragged<float, kRagRmin, kRagRmax>, kRagThreads, p->lds);
+        &per_cu, k_ragged<float, kRagR2min, kRagR2max>, kRagThreads, p->lds);
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &per_cu2, k_ragged<float, kRagR2min, kRagR2max>, kRagThreads, p->lds2);
   }
