@@ -546,7 +546,7 @@ def extra_block(torch, np, ta, N=128, P=65536, reps=5):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1 (batches shard by systems across GPUs with no exchange, so N GPUs run N copies of this workload)",
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import numpy as np
