@@ -698,6 +698,12 @@ EXTERN_C tridStatus_t tridAdiPlanCreate(tridAdiPlan_t *plan, int nx, int ny,
     const int order = dir == 0 ? 0 : 1;
     const int vec = dir == 0 && (nx * ts) % 16 == 0;
     PartCfg c = heuristic(order, n[dir], vec, ts);
+    if (order == 1 && c.kind == 1) {
+      // y and z sweeps without precomputed factors (lines past 512 rows):
+      // the strided batches' layout, most lines per work-group
+      const PartCfg s = strided_default(n[dir], ts);
+      if (s.kind == 1) c = s;
+    }
     if (dir == 0) {
       PartCfg cu = heuristic(order, n[dir], 0, ts);
       cu.idx32 = idx32;
