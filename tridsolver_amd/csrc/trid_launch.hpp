@@ -326,8 +326,8 @@ inline std::vector<PartCfg> candidates(int order, int N, int vec, size_t tsize,
 // past 256 rows; FP32: 8 rows up to 256 rows, else 16, at 1024 threads),
 // with the XCD-contiguous order past 512 rows (FP32: past 128).  From tuning
 // sweeps on MI355X over [256, N, k] batches of 2^24 unknowns: at N = 300-512
-// this moves FP64 from 33 % to 50-61 % of 8 TB/s and FP32 from 31 % to
-// 53-59 %; at N = 1500 FP64 from 8 % to 36 %.  Returns kind 0 when the rule
+// this moves FP64 from 33 % to 46-59 % of 8 TB/s and FP32 from 31 % to
+// 52-58 %; at N = 1500 FP64 from 8 % to 35 %.  Returns kind 0 when the rule
 // does not apply (FP64 up to 256 rows keeps the generic choice).
 inline PartCfg strided_default(int N, size_t tsize) {
   PartCfg c;
