@@ -11,8 +11,11 @@
  * dstar (N*P*M) are caller-provided scratch that receives the forward
  * sweep's C'_n = X_n^{-1} C_n and d'_n.  All pointers are device pointers.
  *
- * The reference declares these with C++ linkage and no status; here they
- * are extern "C".  dbtsv_gpu enqueues without waiting (as the reference),
+ * The reference declares these with C++ linkage and no status.  Here they
+ * are declared extern "C"; the library exports them with C++ linkage too
+ * (the mangled names), so that objects compiled against the reference's
+ * own blktrid_gpu.h link unchanged.  dbtsv_gpu enqueues without waiting
+ * (as the reference),
  * sbtsv_gpu waits for completion (as the reference, without its timing
  * printout).  Errors are reported through tridGetLastError(); the
  * tridXbtsvStridedBatch forms return the status instead.

@@ -30,7 +30,12 @@ typedef struct tridAdiPlan_st *tridAdiPlan_t;
  * sweeps of lines of at most 512 rows use precomputed factors of the
  * constant-coefficient matrix (computed here, once).  With autotuning on
  * (tridSetAutotune), the plan times its kernel choices at its first use, on
- * the arrays of that call, inside the time step, and keeps the fastest. */
+ * the arrays of that call, inside the time step, and keeps the fastest.  A
+ * plan whose first use is inside a caller's stream capture is not tuned (a
+ * capture cannot hold the timing; later tuning would replace the factor
+ * tables that the caller's graph reads).  tridAdiPlanSetSweep and
+ * tridAdiPlanDestroy free factor tables: a caller's graph that holds steps
+ * of the plan must not be launched after them. */
 EXTERN_C tridStatus_t tridAdiPlanCreate(tridAdiPlan_t *plan, int nx, int ny,
                                         int nz, int is_double, double lambda,
                                         int mode);

@@ -55,6 +55,12 @@ EXTERN_C tridStatus_t tridSgtsvRaggedBatch(tridRaggedPlan_t plan,
                                            const float *c, float *d, float *u,
                                            int inc, int sync);
 
+/* No reference counterpart: the number of elements every array of a solve
+ * with this plan must hold, offsets[nsys] (0 for an empty plan).  Callers
+ * that know their arrays' sizes check them against it before a solve. */
+EXTERN_C tridStatus_t tridRaggedPlanExtent(tridRaggedPlan_t plan,
+                                           long long *elements);
+
 /* Summary of the plan's bins ("R8/cps4: 1234 systems, ..."). */
 EXTERN_C const char *tridRaggedPlanDescribe(tridRaggedPlan_t plan);
 

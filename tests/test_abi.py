@@ -88,3 +88,52 @@ def test_no_oracle_in_product_path():
 def test_float_padded_forms_exported(fn):
     import tridsolver_amd
     assert hasattr(tridsolver_amd.lib(), fn)
+
+
+def test_block_cxx_linkage_exports(tmp_path):
+    """An object compiled against declarations with the reference's C++
+    linkage (block/include/blktrid_gpu.h:38-40) references mangled names;
+    the library exports exactly those."""
+    src = tmp_path / "caller.cpp"
+    src.write_text(
+        "void sbtsv_gpu(float *A, float *B, float *C, float *d, float *Cstar, float *dstar,\n"
+        "               float *u, const int N, const int P, const int blkdim);\n"
+        "void dbtsv_gpu(double *A, double *B, double *C, double *d, double *Cstar,\n"
+        "               double *dstar, double *u, const int N, const int P, const int blkdim);\n"
+        "void call(float *f, double *g) {\n"
+        "  sbtsv_gpu(f, f, f, f, f, f, f, 1, 1, 1);\n"
+        "  dbtsv_gpu(g, g, g, g, g, g, g, 1, 1, 1);\n"
+        "}\n")
+    obj = tmp_path / "caller.o"
+    subprocess.check_call(["g++", "-c", str(src), "-o", str(obj)])
+    undef = {ln.split()[-1] for ln in
+             subprocess.check_output(["nm", "-u", str(obj)]).decode().splitlines() if ln.strip()}
+    wanted = {s for s in undef if "btsv_gpu" in s}
+    assert len(wanted) == 2 and all(s.startswith("_Z") for s in wanted), wanted
+    out = subprocess.check_output(["nm", "-D", "--defined-only", LIB]).decode()
+    exported = {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+    assert wanted <= exported, sorted(wanted - exported)
+    assert {"sbtsv_gpu", "dbtsv_gpu"} <= exported  # the extern "C" forms stay
+
+
+def test_ragged_array_size_check():
+    """RaggedPlan.solve refuses arrays shorter than offsets[nsys] (the
+    kernels access every element the offsets name), wrong dtypes and
+    non-contiguous arrays, before any device call."""
+    import torch
+    from tridsolver_amd.ragged import check_arrays
+    f64 = torch.float64
+    ok = [torch.zeros(100, dtype=f64) for _ in range(4)]
+    check_arrays(100, f64, *ok)
+    check_arrays(0, f64, *ok)
+    short = [torch.zeros(100, dtype=f64)] * 3 + [torch.zeros(99, dtype=f64)]
+    with pytest.raises(ValueError, match="d holds 99"):
+        check_arrays(100, f64, *short)
+    with pytest.raises(ValueError, match="dtype"):
+        check_arrays(10, f64, *ok[:3], torch.zeros(100, dtype=torch.float32))
+    with pytest.raises(ValueError, match="contiguous"):
+        check_arrays(10, f64, *ok[:3], torch.zeros(200, dtype=f64)[::2])
+    with pytest.raises(ValueError, match="u is required"):
+        check_arrays(10, f64, *ok, inc=True)
+    with pytest.raises(ValueError, match="u holds 50"):
+        check_arrays(100, f64, *ok, u=torch.zeros(50, dtype=f64), inc=True)

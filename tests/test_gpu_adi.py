@@ -339,3 +339,36 @@ def test_adi_invalid_arguments():
         plan.step(u, u, 1)
     with pytest.raises(TridError):
         plan.replay()
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_adi_benchmark_size_512(dt):
+    """The 512^3 grid of the ADI configuration: 20 steps, lambda = 1,
+    a = c = -lambda, b = 1 + 2 lambda, Dirichlet-zero boundary, interior
+    U(0,1) from seed 0, against the oracle's ADI over the same steps
+    (apps/adi/src/adi_cuda.cu:285-330 is the reference driver)."""
+    n = 512
+    u0 = rand_init(n, n, n, dt, seed=0)
+    got, plan = run_gpu(u0, 20, 1)
+    exp = u0.ravel()
+    del u0
+    assert oracle.adi(exp, n, n, n, 20, 1.0, 1) == 0
+    err = oracle.rel_err(exp, got.ravel())
+    print(f"ADI 512^3 {np.dtype(dt).name} 20 steps: rel err vs oracle {err:.2e} [{plan.describe()}]")
+    assert err <= tol(dt)
+
+
+# y and z lines of more than 512 rows run the generic sweeps with the strided
+# default layout (up to 8192 rows); 2100, 4100 and 8192 rows take 16-row
+# chunks with 256 or 512 chunks per line
+@pytest.mark.parametrize("shape", [(4, 2100, 5), (5, 4, 4100), (3, 4, 8192), (3, 1100, 4)])
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_adi_long_y_z_lines(shape, dt, mode):
+    nx, ny, nz = shape
+    u0 = rand_init(nx, ny, nz, dt, seed=nx * ny + nz) if mode == 1 else ref_init(nx, ny, nz, dt)
+    got, plan = run_gpu(u0, 3, mode)
+    exp = u0.ravel().copy()
+    assert oracle.adi(exp, nx, ny, nz, 3, 1.0, mode) == 0
+    err = oracle.rel_err(exp, got.ravel())
+    assert err <= tol(dt), f"{shape} mode {mode}: {err:.2e} [{plan.describe()}]"

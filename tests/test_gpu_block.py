@@ -6,9 +6,12 @@ diagonally dominant), rhs U(-1,1), A_0 = C_{N-1} = 0.  Each solve is
 compared with the oracle's block Thomas (restating block/src/blktrid_cpu.cpp
 with its product index fixed; pinned against dense LU in test_oracle.py):
 normwise relative error <= 1e-12 (FP64), 1e-5 (FP32).  At the benchmark
-size (P = 65536, N = 128) the residual ||A u - d|| / ||d|| is checked
-instead.  The adjoint is checked against dense transposed solves and the
-dot-product identity.
+size (P = 65536, N = 128, every M of the block configuration) the inputs
+are generated on the device; every system's residual ||K u - d|| / ||d||
+is checked there, and a sample of systems against the oracle.  The adjoint
+is checked against dense transposed solves and the dot-product identity,
+and at the configuration's size through the transposed residual, the dot
+identity, its outer-product gradients and a sample against the oracle.
 """
 from __future__ import annotations
 
@@ -125,20 +128,121 @@ def test_block_adjoint(M):
     assert abs(np.sum(u * g) - np.sum(d * got[3])) <= 1e-12 * np.sum(np.abs(u * g))
 
 
-@pytest.mark.parametrize("M", [2, 3, 4, 5, 8])
-def test_block_benchmark_size_residual(M):
-    """N = 128 block rows, P = 65536 systems (fewer for M >= 5 to keep the
-    host-side generation of the inputs within seconds; the benchmark runs
-    the full batch): residual ||A u - d|| / ||d||."""
+def bench_inputs(N, P, M, seed):
+    """The block configuration's inputs, generated on the device (torch
+    Philox, `seed`): off-diagonal blocks U(-1,1)/M, B = U(-1,1)/M +
+    (row-sum |A| + |C| + 1) I, d ~ U(-1,1), A_0 = C_{N-1} = 0, layout
+    (N, P, M, M)."""
     import torch
-    N, P = 128, {2: 65536, 3: 65536, 4: 65536, 5: 32768, 8: 8192}[M]
-    A, B, C, d = blk_system(M, N, P, seed=M)
-    tA, tB, tC, td = (dev(x) for x in (A, B, C, d))
-    del A, B, C
-    u = blk.btsv(tA, tB, tC, td)
-    r = torch.matmul(tB, u.unsqueeze(-1)).squeeze(-1)
-    r[1:] += torch.matmul(tA[1:], u[:-1].unsqueeze(-1)).squeeze(-1)
-    r[:-1] += torch.matmul(tC[:-1], u[1:].unsqueeze(-1)).squeeze(-1)
-    rel = float((r - td).abs().max() / td.abs().max())
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+
+    def U(*shape):
+        return torch.rand(*shape, generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+    A = U(N, P, M, M) / M
+    Cm = U(N, P, M, M) / M
+    A[0].zero_()
+    Cm[N - 1].zero_()
+    B = U(N, P, M, M) / M
+    B += torch.diag_embed(A.abs().sum(-1) + Cm.abs().sum(-1) + 1.0)
+    d = U(N, P, M)
+    return A, B, Cm, d
+
+
+def apply_blocks(A, B, Cm, u, transpose=False):
+    """K u for the block-tridiagonal K of (A, B, C), or K^T u: elementwise
+    products summed on the device (no library GEMM)."""
+    if transpose:
+        r = (B * u.unsqueeze(-1)).sum(-2)
+        r[1:] += (Cm[:-1] * u[:-1].unsqueeze(-1)).sum(-2)
+        r[:-1] += (A[1:] * u[1:].unsqueeze(-1)).sum(-2)
+    else:
+        r = (B * u.unsqueeze(-2)).sum(-1)
+        r[1:] += (A[1:] * u[:-1].unsqueeze(-2)).sum(-1)
+        r[:-1] += (Cm[:-1] * u[1:].unsqueeze(-2)).sum(-1)
+    return r
+
+
+SAMPLE = [0, 1, 2, 31, 32, 63, 64, 1000, 32767, 32768, 65534, 65535]
+
+
+@pytest.mark.parametrize("M", [2, 3, 4, 5, 8])
+def test_block_benchmark_size(M):
+    """The block configuration at full size: N = 128 block rows, P = 65536
+    systems, FP64 (reference solve block/src/blktrid_gpu.cu:57-179).  Every
+    system: residual ||K u - d||_inf / ||d||_inf <= 1e-12 on the device; a
+    sample of systems (both ends of the batch, wave and group boundaries):
+    the oracle's block Thomas, normwise relative error <= 1e-12."""
+    N, P = 128, 65536
+    A, B, Cm, d = bench_inputs(N, P, M, seed=M)
+    u = blk.btsv(A, B, Cm, d)
+    rel = float((apply_blocks(A, B, Cm, u) - d).abs().max() / d.abs().max())
     print(f"block M={M} P={P} N={N}: residual {rel:.2e}")
     assert rel <= 1e-12
+    h = [x[:, SAMPLE].contiguous().cpu().numpy() for x in (A, B, Cm, d, u)]
+    exp = oracle.btsv(h[0], h[1], h[2], h[3], N, len(SAMPLE), M, layout=0)
+    err = oracle.rel_err(exp, h[4])
+    print(f"block M={M}: sampled systems vs oracle {err:.2e}")
+    assert err <= 1e-12
+
+
+@pytest.mark.parametrize("M", [2, 3, 4, 5, 8])
+def test_block_adjoint_benchmark_size(M):
+    """The block adjoint at the configuration's size (N = 128, P = 65536):
+    lambda = grad_d solves K^T lambda = g (device residual <= 1e-12), the
+    dot identity <K^{-1} d, g> = <d, K^{-T} g>, the outer-product gradients
+    grad_B_n = -lambda_n u_n^T, grad_A_n = -lambda_n u_{n-1}^T, grad_C_n =
+    -lambda_n u_{n+1}^T, and a sample of systems against the oracle's block
+    Thomas on the transposed blocks (C_{n-1}^T, B_n^T, A_{n+1}^T)."""
+    import torch
+    N, P = 128, 65536
+    A, B, Cm, d = bench_inputs(N, P, M, seed=50 + M)
+    u = blk.btsv(A, B, Cm, d)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(7)
+    g = torch.rand(N, P, M, generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+    gA, gB, gC, lam = blk.btsv_adjoint(A, B, Cm, u, g)
+    rel = float((apply_blocks(A, B, Cm, lam, transpose=True) - g).abs().max() / g.abs().max())
+    print(f"block adjoint M={M}: transposed residual {rel:.2e}")
+    assert rel <= 1e-12
+    lhs, rhs = float((u * g).sum()), float((d * lam).sum())
+    assert abs(lhs - rhs) <= 1e-12 * float((u * g).abs().sum())
+    outer = -lam.unsqueeze(-1) * u.unsqueeze(-2)
+    assert float((gB - outer).abs().max()) <= 1e-15 * float(outer.abs().max())
+    assert float(gA[0].abs().max()) == 0.0 and float(gC[N - 1].abs().max()) == 0.0
+    assert torch.equal(gA[1:], -lam[1:].unsqueeze(-1) * u[:-1].unsqueeze(-2))
+    assert torch.equal(gC[:-1], -lam[:-1].unsqueeze(-1) * u[1:].unsqueeze(-2))
+    h = [x[:, SAMPLE].contiguous().cpu().numpy() for x in (A, B, Cm, g, lam)]
+    At = np.zeros_like(h[0])
+    Ct = np.zeros_like(h[0])
+    At[1:] = np.swapaxes(h[2][:-1], -1, -2)
+    Ct[:-1] = np.swapaxes(h[0][1:], -1, -2)
+    Bt = np.ascontiguousarray(np.swapaxes(h[1], -1, -2))
+    exp = oracle.btsv(At, Bt, Ct, h[3], N, len(SAMPLE), M, layout=0)
+    err = oracle.rel_err(exp, h[4])
+    print(f"block adjoint M={M}: sampled lambda vs oracle {err:.2e}")
+    assert err <= 1e-12
+
+
+def test_block_mixed_benchmark_size():
+    """All five block sizes of the configuration (M = 2, 3, 4, 5, 8; N = 128,
+    P = 65536 each) in one mixed launch; every group agrees with its own
+    separate solve within 1e-12 and its residual is <= 1e-12."""
+    import torch
+    N, P = 128, 65536
+    groups = []
+    for M in (2, 3, 4, 5, 8):
+        A, B, Cm, d = bench_inputs(N, P, M, seed=90 + M)
+        g = dict(A=A, B=B, C=Cm, d=d, Cstar=torch.empty_like(A), dstar=torch.empty_like(d),
+                 u=torch.empty_like(d))
+        groups.append(g)
+    blk.btsv_mixed(groups)
+    for g in groups:
+        M = g["d"].shape[-1]
+        ref = blk.btsv(g["A"], g["B"], g["C"], g["d"])
+        diff = float((ref - g["u"]).abs().max() / ref.abs().max())
+        rel = float((apply_blocks(g["A"], g["B"], g["C"], g["u"]) - g["d"]).abs().max()
+                    / g["d"].abs().max())
+        print(f"mixed M={M}: vs separate solve {diff:.2e}, residual {rel:.2e}")
+        assert diff <= 1e-12 and rel <= 1e-12
+        del ref

@@ -563,14 +563,14 @@ tridStatus_t btsv(const T *A, const T *B, const T *C, const T *d, T *Cs, T *ds,
   tridStatus_t s = check_blk(A, B, C, d, u, N, P, M);
   if (s) return s;
   if (N == 0 || P == 0) return TRID_STATUS_SUCCESS;
+  hipStream_t st = current_stream();
   if (!Cs || !ds) {
     size_t mat = (size_t)N * P * M * M, v = (size_t)N * P * M;
-    T *w = static_cast<T *>(workspace((mat + v) * sizeof(T)));
-    if (!w) return fail(TRID_STATUS_ALLOC_FAILED, "workspace allocation failed");
+    T *w = static_cast<T *>(workspace((mat + v) * sizeof(T), st));
+    if (!w) return TRID_STATUS_ALLOC_FAILED;  // reason recorded by workspace()
     Cs = w;
     ds = w + mat;
   }
-  hipStream_t st = current_stream();
   BlkArgs<T> g{A, B, C, d, Cs, ds, u, N, P};
   s = check_hip(launch_blk<T, 0>(g, M, g_force_lanes.load(), st), "block kernel launch");
   if (s) return s;
@@ -665,9 +665,9 @@ EXTERN_C tridStatus_t tridDbtsvAdjoint(const double *A, const double *B,
     return fail(TRID_STATUS_INVALID_VALUE, "NULL gradient array");
   if (N == 0 || P == 0) return TRID_STATUS_SUCCESS;
   size_t mat = (size_t)N * P * M * M, v = (size_t)N * P * M;
-  double *w = static_cast<double *>(workspace((mat + v) * sizeof(double)));
-  if (!w) return fail(TRID_STATUS_ALLOC_FAILED, "workspace allocation failed");
   hipStream_t st = current_stream();
+  double *w = static_cast<double *>(workspace((mat + v) * sizeof(double), st));
+  if (!w) return TRID_STATUS_ALLOC_FAILED;  // reason recorded by workspace()
   // lambda = A^{-T} g  (written straight into grad_d)
   BlkArgs<double> ga{A, B, C, g, w, w + mat, grad_d, N, P};
   s = check_hip(launch_blk<double, 1>(ga, M, g_force_lanes.load(), st), "adjoint kernel launch");

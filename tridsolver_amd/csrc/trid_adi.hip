@@ -651,12 +651,15 @@ std::string plan_text(const tridAdiPlan_st *p) {
          " y:" + sweep_name(1) + " z:" + sweep_name(2);
 }
 
-// With autotuning on, a plan tunes at its first use, on that call's arrays
-// (never inside a graph capture: tridAdiGraphCapture tunes before it starts
-// capturing).
+// With autotuning on, a plan tunes at its first use, on that call's arrays,
+// and never inside a graph capture: tridAdiGraphCapture tunes before it
+// starts capturing.  A plan whose first use is on a stream that a caller is
+// capturing stays untuned: tuning replaces the factor tables, whose
+// addresses the caller's graph holds.
 void ensure_tuned(tridAdiPlan_st *p, void *u, void *du) {
   if (!p->tune_pending) return;
   p->tune_pending = false;
+  if (capturing(current_stream())) return;
   if (p->dbl)
     tune_plan_t<double>(p, static_cast<double *>(u), static_cast<double *>(du));
   else
@@ -699,8 +702,10 @@ EXTERN_C tridStatus_t tridAdiPlanCreate(tridAdiPlan_t *plan, int nx, int ny,
     const int vec = dir == 0 && (nx * ts) % 16 == 0;
     PartCfg c = heuristic(order, n[dir], vec, ts);
     if (order == 1 && c.kind == 1) {
-      // y and z sweeps without precomputed factors (lines past 512 rows):
-      // the strided batches' layout, most lines per work-group
+      // y and z sweeps that run without precomputed factors (by default,
+      // lines past 512 rows; any line when the plan's sweep is pinned to
+      // rows = 0 or the tuner picks a non-factor candidate): the strided
+      // batches' layout, most lines per work-group
       const PartCfg s = strided_default(n[dir], ts);
       if (s.kind == 1) c = s;
     }
@@ -827,12 +832,8 @@ EXTERN_C tridStatus_t tridAdiStep(tridAdiPlan_t p, void *u, void *du, int nsteps
   // removes the launch gaps between the kernels (a one-off call does not
   // pay for a capture); the first call, the remainder and calls on a stream
   // that is itself capturing are launched kernel by kernel
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  const hipError_t qe = hipStreamIsCapturing(st, &cs);
-  if (qe != hipSuccess) (void)hipGetLastError();
-  const bool capturing = qe != hipSuccess || cs != hipStreamCaptureStatusNone;
   int rest = nsteps;
-  if (nsteps >= 2 && !capturing) {
+  if (nsteps >= 2 && !capturing(st)) {
     const int chunk = std::min(nsteps, kStepGraphMax);
     if (p->step_u == u && p->step_du == du && p->step_n == chunk) {
       if (!p->step_exec) {

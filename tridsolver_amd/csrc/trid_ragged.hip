@@ -303,6 +303,7 @@ using namespace tridhip;
 struct tridRaggedPlan_st {
   int dbl = 1;
   long long nsys = 0;
+  long long extent = 0;  // offsets[nsys]: elements every array must hold
   const int64_t *offsets = nullptr;  // caller's device array
   int *sys = nullptr;
   RagItem *items = nullptr;
@@ -376,8 +377,8 @@ tridStatus_t ragged_solve(tridRaggedPlan_st *p, const T *a, const T *b, const T 
     if (s) return s;
   }
   if (p->nlong > 0) {
-    T *ws = static_cast<T *>(workspace((size_t)p->long_ws * sizeof(T)));
-    if (!ws) return fail(TRID_STATUS_ALLOC_FAILED, "workspace allocation failed");
+    T *ws = static_cast<T *>(workspace((size_t)p->long_ws * sizeof(T), st));
+    if (!ws) return TRID_STATUS_ALLOC_FAILED;  // reason recorded by workspace()
     hipLaunchKernelGGL(k_ragged_long<T>, dim3((p->nlong + 63) / 64), dim3(64), 0, st, g,
                        p->long_ids, p->long_ws_off, p->nlong, ws);
     tridStatus_t s = check_hip(hipGetLastError(), "ragged long-system launch");
@@ -432,11 +433,16 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
     free_plan(p);
     return s;
   }
+  if (off[0] < 0) {
+    free_plan(p);
+    return fail(TRID_STATUS_INVALID_VALUE, "offsets[0] must be >= 0");
+  }
   for (long long k = 0; k < nsys; k++)
     if (off[k + 1] < off[k]) {
       free_plan(p);
       return fail(TRID_STATUS_INVALID_VALUE, "offsets must be non-decreasing");
     }
+  p->extent = off[nsys];
   // bins (cps = 2^cl >= L / rmax, R = the shortest of rmin..rmax rows with
   // cps*R >= L): FP64 chunks of 5..8 rows; FP32 (half the registers per
   // row) chunks of 9..16 rows, half the chunks and reduced-system levels
@@ -617,6 +623,13 @@ EXTERN_C tridStatus_t tridSgtsvRaggedBatch(tridRaggedPlan_t plan,
                                            const float *c, float *d, float *u,
                                            int inc, int sync) {
   return ragged_solve<float>(plan, a, b, c, d, u, inc, sync);
+}
+
+EXTERN_C tridStatus_t tridRaggedPlanExtent(tridRaggedPlan_t plan, long long *elements) {
+  clear_error();
+  if (!plan || !elements) return fail(TRID_STATUS_INVALID_VALUE, "plan or elements is NULL");
+  *elements = plan->extent;
+  return TRID_STATUS_SUCCESS;
 }
 
 EXTERN_C const char *tridRaggedPlanDescribe(tridRaggedPlan_t plan) {

@@ -5,8 +5,10 @@
 
 #include <atomic>
 #include <cstdlib>
+#include <map>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/trid_hip_ext.h"
@@ -25,8 +27,13 @@ thread_local std::string t_config;
 struct DevWs {
   void *ptr = nullptr;
   size_t bytes = 0;
+  bool captured = false;  // handed to a call during a stream capture
 };
-std::vector<DevWs> g_ws;
+// keyed by (device, stream)
+std::map<std::pair<int, hipStream_t>, DevWs> g_ws;
+// buffers replaced after a capture used them: a caller's graph may still
+// launch kernels that read them, so they are kept until the process ends
+std::vector<void *> g_retired;
 }  // namespace
 
 hipStream_t current_stream() { return g_stream.load(); }
@@ -50,22 +57,50 @@ tridStatus_t check_hip(hipError_t e, const char *what) {
   return fail(TRID_STATUS_EXECUTION_FAILED, m);
 }
 
-void *workspace(size_t bytes) {
+bool capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
+    (void)hipGetLastError();
+    return true;
+  }
+  return cs != hipStreamCaptureStatusNone;
+}
+
+void *workspace(size_t bytes, hipStream_t st) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
   std::lock_guard<std::mutex> lk(g_mu);
-  if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1);
-  DevWs &w = g_ws[dev];
-  if (w.bytes >= bytes && w.ptr) return w.ptr;
+  DevWs &w = g_ws[std::make_pair(dev, st)];
+  const bool cap = capturing(st);
+  if (w.bytes >= bytes && w.ptr) {
+    w.captured |= cap;
+    return w.ptr;
+  }
+  if (cap) {
+    fail(TRID_STATUS_ALLOC_FAILED,
+         "workspace of " + std::to_string(bytes) +
+             " bytes cannot be allocated during stream capture: run the same call once "
+             "before capturing it");
+    return nullptr;
+  }
   if (w.ptr) {
-    // the old buffer may still be read by queued kernels of this library
-    (void)hipDeviceSynchronize();
-    (void)hipFree(w.ptr);
+    if (w.captured) {
+      g_retired.push_back(w.ptr);
+    } else {
+      // the old buffer may still be read by queued kernels on this stream
+      (void)hipStreamSynchronize(st);
+      (void)hipFree(w.ptr);
+    }
     w.ptr = nullptr;
     w.bytes = 0;
+    w.captured = false;
   }
   void *p = nullptr;
-  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    fail(TRID_STATUS_ALLOC_FAILED, "workspace allocation of " + std::to_string(bytes) +
+                                       " bytes failed");
+    return nullptr;
+  }
   w.ptr = p;
   w.bytes = bytes;
   return p;

@@ -21,9 +21,20 @@ void set_last_config(const std::string &cfg);
 // Maps a HIP error to a status (and records it); hipSuccess -> SUCCESS.
 tridStatus_t check_hip(hipError_t e, const char *what);
 
-// Grow-only device workspace of the current device (for the fallback
-// kernel and autotuning scratch).  Returns nullptr on allocation failure.
-void *workspace(size_t bytes);
+// True when work enqueued on `st` would be recorded into a graph rather
+// than run: the stream itself is capturing, or the query is refused (the
+// null stream while another stream captures in global mode).
+bool capturing(hipStream_t st);
+
+// Grow-only device workspace of the current device and stream `st`: calls
+// on different streams never share a buffer, calls on one stream run in
+// order.  Growing waits for `st` and frees the old buffer, except a buffer
+// that a call under capture used: a caller's graph keeps its address, so it
+// is kept until the process ends.  During a capture on `st` no allocation
+// is made: a big-enough buffer from an earlier call is returned, otherwise
+// nullptr with the reason recorded as the last error.  Returns nullptr on
+// allocation failure.
+void *workspace(size_t bytes, hipStream_t st);
 
 int autotune_mode();
 

@@ -277,20 +277,33 @@ long long extent(int ndim, const int *dims, const int *pads) {
   return e + 1;
 }
 
+// Runs one configuration.  Scratch of the fallback and split-line kinds
+// comes from the stream's workspace, or, when `fixed` is given, from that
+// buffer of `fixed_bytes`, which must hold it.
 template <typename T, int INC>
 tridStatus_t run_cfg(const PartCfg &cfg, const T *a, const T *b, const T *c,
-                     T *d, T *u, const SysDesc &D, hipStream_t st) {
+                     T *d, T *u, const SysDesc &D, hipStream_t st, void *fixed = nullptr,
+                     size_t fixed_bytes = 0) {
   hipError_t e;
-  if (cfg.kind == 2) {
-    size_t bytes = 2 * (size_t)D.N * (size_t)D.nsys * sizeof(T);
-    T *ws = static_cast<T *>(workspace(bytes));
-    if (!ws) return fail(TRID_STATUS_ALLOC_FAILED, "workspace allocation failed");
-    e = launch_fallback<T, INC>(a, b, c, d, u, D, st, ws);
-  } else if (cfg.kind == 4) {
-    const long long el = split_ws_elems(cfg, D.N, D.nsys, sizeof(T));
-    T *ws = static_cast<T *>(workspace((size_t)el * sizeof(T)));
-    if (!ws) return fail(TRID_STATUS_ALLOC_FAILED, "workspace allocation failed");
-    e = split_solve<T, INC>(cfg, a, b, c, d, u, D, st, ws);
+  if (cfg.kind == 2 || cfg.kind == 4) {
+    const size_t bytes =
+        cfg.kind == 2 ? 2 * (size_t)D.N * (size_t)D.nsys * sizeof(T)
+                      : (size_t)split_ws_elems(cfg, D.N, D.nsys, sizeof(T)) * sizeof(T);
+    T *ws;
+    if (fixed) {
+      if (bytes > fixed_bytes)
+        return fail(TRID_STATUS_INVALID_VALUE,
+                    "this solve needs " + std::to_string(bytes) + " bytes of scratch; the "
+                    "no-allocation entry point holds " + std::to_string(fixed_bytes));
+      ws = static_cast<T *>(fixed);
+    } else {
+      ws = static_cast<T *>(workspace(bytes, st));
+      if (!ws) return TRID_STATUS_ALLOC_FAILED;  // reason recorded by workspace()
+    }
+    if (cfg.kind == 2)
+      e = launch_fallback<T, INC>(a, b, c, d, u, D, st, ws);
+    else
+      e = split_solve<T, INC>(cfg, a, b, c, d, u, D, st, ws);
   } else {
     e = launch_part<T, INC>(cfg, CoefArrays<T>{a, b, c}, d, u, D, st);
   }
@@ -313,7 +326,7 @@ PartCfg autotune(int order, int vec, bool gvec, int idx32, bool dense, const T *
   // large power-of-two strides depend on the actual addresses), repeatedly
   // and in place as a caller runs them; the written array is saved first and
   // restored at the end
-  T *saved = static_cast<T *>(workspace((size_t)d_extent * sizeof(T)));
+  T *saved = static_cast<T *>(workspace((size_t)d_extent * sizeof(T), st));
   if (!saved) return best;
   T *dst = INC ? u : d;
   const size_t bytes = (size_t)d_extent * sizeof(T);
@@ -332,6 +345,34 @@ PartCfg autotune(int order, int vec, bool gvec, int idx32, bool dense, const T *
               order, cfg_name(cands[c], sizeof(T)).c_str(), tm[c]);
   (void)hipMemcpyAsync(dst, saved, bytes, hipMemcpyDeviceToDevice, st);
   return best;
+}
+
+// Untuned configuration of a strided batch: the heuristic chunk layout,
+// the strided default for solvedim >= 1, wave staging where it applies and
+// the chunk length that pads a solvedim-0 system least.
+template <typename T>
+PartCfg default_cfg(int order, const SysDesc &D, int vec, bool gvec, bool dense) {
+  PartCfg cfg = heuristic(order, D.N, vec, sizeof(T));
+  if (order == 1 && cfg.kind == 1) {
+    const PartCfg s = strided_default(D.N, sizeof(T));
+    if (s.kind == 1) cfg = s;
+  }
+  // staging pays when a chunk is more than one 16-byte vector
+  if (cfg.R * sizeof(T) > 16) {
+    if (stage_ok(cfg, dense, D.nsys, D.N)) {
+      cfg.vec = staged_vec(cfg);
+    } else {
+      // general staging takes up to 512 threads: longer FP64 chunks if
+      // needed (FP32 is faster unstaged than in 16-row chunks)
+      PartCfg c16 = make_cfg(order, 16, D.N, cfg.vec, 256);
+      if (sizeof(T) == 8 && !gstage_ok(cfg, gvec, dense) && cfg_valid(c16, D.N) &&
+          gstage_ok(c16, gvec, dense))
+        cfg = c16;
+      if (gstage_ok(cfg, gvec, dense)) cfg.vec = 4;
+    }
+  }
+  // solvedim 0 between powers of two: 12-row chunks when they pad less
+  return pick_rows(cfg, D.N, gvec, dense, sizeof(T));
 }
 
 template <typename T, int INC>
@@ -450,12 +491,17 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
     if (fc.stage && stage_ok(cfg, dense, D.nsys, D.N)) cfg.vec = staged_vec(cfg);
     else if (fc.stage && gstage_ok(cfg, gvec, dense)) cfg.vec = 4;
     cfg.xcd = fc.kind == 3;
-  } else if (autotune_mode() == 1) {
+  } else {
+    // tuned configurations are cached per shape; a shape first seen while
+    // `st` is being captured takes the untuned default (tuning synchronises
+    // and times launches, which a capture cannot hold) and is tuned at its
+    // first call outside a capture
+    bool have = false;
+    const bool tune = autotune_mode() == 1;
     auto key = std::make_tuple((int)sizeof(T), INC, order, D.N, D.nsys,
                                vec * 4 + (int)gvec * 2 + (int)dense, D.sstr[3],
                                D.nother > 0 ? D.ostr[3][0] : 0LL);
-    bool have = false;
-    {
+    if (tune) {
       std::lock_guard<std::mutex> lk(g_tune_mu);
       auto it = g_tuned.find(key);
       if (it != g_tuned.end()) {
@@ -463,35 +509,14 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
         have = true;
       }
     }
-    if (!have) {
+    if (!have && tune && !capturing(st)) {
       long long ext = extent(ndim, dims, INC ? pads[4] : pads[3]);
-      cfg = autotune<T, INC>(order, vec, gvec, idx32, dense, a, b, c, d, u, D, ext,
-                             st);
+      cfg = autotune<T, INC>(order, vec, gvec, idx32, dense, a, b, c, d, u, D, ext, st);
       std::lock_guard<std::mutex> lk(g_tune_mu);
       g_tuned[key] = cfg;
+      have = true;
     }
-  } else {
-    cfg = heuristic(order, D.N, vec, sizeof(T));
-    if (order == 1 && cfg.kind == 1) {
-      const PartCfg s = strided_default(D.N, sizeof(T));
-      if (s.kind == 1) cfg = s;
-    }
-    // staging pays when a chunk is more than one 16-byte vector
-    if (cfg.R * sizeof(T) > 16) {
-      if (stage_ok(cfg, dense, D.nsys, D.N)) {
-        cfg.vec = staged_vec(cfg);
-      } else {
-        // general staging takes up to 512 threads: longer FP64 chunks if
-        // needed (FP32 is faster unstaged than in 16-row chunks)
-        PartCfg c16 = make_cfg(order, 16, D.N, cfg.vec, 256);
-        if (sizeof(T) == 8 && !gstage_ok(cfg, gvec, dense) && cfg_valid(c16, D.N) &&
-            gstage_ok(c16, gvec, dense))
-          cfg = c16;
-        if (gstage_ok(cfg, gvec, dense)) cfg.vec = 4;
-      }
-    }
-    // solvedim 0 between powers of two: 12-row chunks when they pad less
-    cfg = pick_rows(cfg, D.N, gvec, dense, sizeof(T));
+    if (!have) cfg = default_cfg<T>(order, D, vec, gvec, dense);
   }
   // systems longer than one work-group holds: split-line solve.  Strided
   // systems (ORDER 1) of 2048 rows and more take it too: a work-group that

@@ -482,9 +482,9 @@ EXTERN_C tridStatus_t tridVadiForward(const double *kappa, double *u, int nx, in
   if (nx < 1 || ny < 1 || nz < 1 || steps < 0 || !kappa || !u)
     return fail(TRID_STATUS_INVALID_VALUE, "bad vadi arguments");
   const Grid g = make_grid(nx, ny, nz);
-  double *r = static_cast<double *>(workspace((size_t)g.n * sizeof(double)));
-  if (!r) return fail(TRID_STATUS_ALLOC_FAILED, "workspace allocation failed");
   hipStream_t st = current_stream();
+  double *r = static_cast<double *>(workspace((size_t)g.n * sizeof(double), st));
+  if (!r) return TRID_STATUS_ALLOC_FAILED;  // reason recorded by workspace()
   for (int s = 0; s < steps; s++) {
     tridStatus_t e = vadi_step(kappa, u, u, r, r, g, lam, st);
     if (e) return e;
@@ -512,9 +512,10 @@ EXTERN_C tridStatus_t tridVadiLossGrad(const double *kappa, const double *u0,
   const long long na = (n + 31) / 32 * 32;
   const long long nblk = blocks_for(n);
   const size_t arrays = (size_t)(nseg - 1) + 1 + (size_t)ckpt + 2 * (size_t)ckpt + 3 + 1;
+  hipStream_t st = current_stream();
   double *ws =
-      static_cast<double *>(workspace((arrays * na + nblk + 1) * sizeof(double)));
-  if (!ws) return fail(TRID_STATUS_ALLOC_FAILED, "workspace allocation failed");
+      static_cast<double *>(workspace((arrays * na + nblk + 1) * sizeof(double), st));
+  if (!ws) return TRID_STATUS_ALLOC_FAILED;  // reason recorded by workspace()
   double *cur = ws;
   auto take = [&]() {
     double *p = cur;
@@ -531,7 +532,6 @@ EXTERN_C tridStatus_t tridVadiLossGrad(const double *kappa, const double *u0,
   for (int i = 0; i < ckpt; i++) Q[i] = take();
   double *LZ = take(), *LY = take(), *LX = take(), *r = take();
   double *partial = cur;
-  hipStream_t st = current_stream();
   tridStatus_t s = TRID_STATUS_SUCCESS;
 
   // records steps k0..k1-1 from u_{k0} = us[0]: us[i+1], P[i], Q[i]
