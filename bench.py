@@ -19,9 +19,15 @@ events recorded on the library's stream around every launch.
 "roofline" reports the dominant kernel's achieved algorithmic bandwidth
 (bytes every launch must move at minimum) against MI355X's 8 TB/s HBM peak;
 "traffic" is the HBM bytes per launch that rocprofv3 counters measured for
-that kernel (profiles/traffic.json).  "cpu_baseline" times the CPU oracle (a
-C/OpenMP port of the reference CPU solver) on the host cores on a bounded
-sample.  "extras" (same line) holds the other measured workloads:
+that kernel in the same configuration (profiles/traffic.json; null when
+this run's kernel configuration differs from the measured one).
+"cpu_baseline" times the CPU oracle (a C/OpenMP port of the reference CPU
+solver) on the host cores, OMP_PROC_BIND=close OMP_PLACES=cores, on the same
+workload, and states every result's error against the FP64 oracle
+(err_vs_fp64_oracle: normwise relative error); the oracle runs only in this
+leg, after the GPU timing, never inside a timed region.  "extras" (same
+line) holds the other measured workloads, each with its own CPU baseline and
+error against the FP64 oracle:
   plain     256^3 solves per solvedim, FP64 and FP32;
   ragged    200,000 systems, lengths floor(exp(U(ln 8, ln 4097))) (seed 0),
             one persistent launch vs one strided-batch launch per distinct
@@ -44,6 +50,11 @@ import os
 import sys
 import time
 
+# the CPU baseline's OpenMP threads: one per core, packed (set before any
+# OpenMP runtime starts)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -60,7 +71,6 @@ def parse():
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
     ap.add_argument("--extras", default="plain,ragged,vadi,block,split,adi_sizes",
@@ -70,18 +80,37 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_from_profiles(kernel_key: str):
+def traffic_from_profiles(kernel_key: str, config: str):
     """HBM bytes per launch of a kernel as measured by rocprofv3 --pmc FETCH_SIZE and
     WRITE_SIZE (calibrated against a plain device copy), from profiles/traffic.json;
-    keys are "<workload>|<kernel>|<dtype>"."""
+    keys are "<workload>|<kernel>|<dtype>".  Returns (bytes or None, note): the
+    entry counts only when it was measured on the configuration this run used."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p) as f:
             t = json.load(f)
         v = t.get(kernel_key)
-        return None if v is None else float(v["bytes_per_launch"])
+        if v is None:
+            return None, "no counter measurement for this kernel"
+        if v.get("config") != config:
+            return None, (f"counters were measured on {v.get('config')!r}; this run used "
+                          f"{config!r}")
+        return float(v["bytes_per_launch"]), f"rocprofv3 counters of {v.get('kernel')}"
     except (OSError, ValueError, KeyError):
-        return None
+        return None, "profiles/traffic.json unreadable"
+
+
+def stage_config(desc: str, stage: str) -> str:
+    """The configuration of one ADI stage from the plan's description
+    ("rhs+x(fused,R8)<double> x:... y:... z:...")."""
+    toks = desc.split()
+    pre = {"sweep_x": "x:", "sweep_y": "y:", "sweep_z_inc": "z:"}.get(stage)
+    if pre is None:
+        return toks[0] if toks else ""
+    for t in toks:
+        if t.startswith(pre):
+            return t
+    return ""
 
 
 def plain_workload(args, torch, np, api, ta):
@@ -115,9 +144,16 @@ def plain_workload(args, torch, np, api, ta):
                 events[sd][1].record()
             kernel_cfg[sd] = ta.last_config()
 
+    def result():
+        """one step (the three solves in turn) from the original d, on the host"""
+        dev[3].copy_(torch.from_numpy(host[3]).to("cuda"))
+        step()
+        torch.cuda.synchronize()
+        return dev[3].cpu().numpy()
+
     return dict(step=step, kernels=("solvedim0", "solvedim1", "solvedim2"),
                 unknowns_per_step=3 * unknowns, algo_bytes=[algo_bytes] * 3,
-                host=host, dims=dims, kernel_cfg=kernel_cfg,
+                host=host, dims=dims, kernel_cfg=kernel_cfg, result=result,
                 workload=f"plain batched scalar solve {n}^3 along solvedim 0,1,2")
 
 
@@ -132,6 +168,7 @@ def adi_workload(args, torch, np, api, ta):
     u = torch.from_numpy(u0).to("cuda")
     du = torch.empty_like(u)
     plan = AdiPlan(n, n, n, tdt, 1.0, 1)
+    u0_dev = u.clone()
     pts = n ** 3
     fused = "fused" in plan.describe()
     # algorithmic bytes per launch: rhs (+x when fused) reads u and writes du;
@@ -166,49 +203,116 @@ def adi_workload(args, torch, np, api, ta):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / k
 
+    def result(k):
+        """k eager steps (the timed stages) from u0, on the host"""
+        u.copy_(u0_dev)
+        for _ in range(k):
+            step()
+        torch.cuda.synchronize()
+        return u.cpu().numpy()
+
     return dict(step=step, kernels=names,
                 unknowns_per_step=3 * pts, algo_bytes=algo, host=[u0], dims=[n, n, n],
-                kernel_cfg={}, describe=plan.describe, graph_ms=graph_ms,
+                kernel_cfg={}, describe=plan.describe, graph_ms=graph_ms, result=result,
                 workload=f"ADI heat diffusion {n}^3, lambda=1, a=c=-lambda, b=1+2lambda, "
                          f"Dirichlet-zero, u0~U(0,1) seed 0 (apps/adi time step)")
 
 
-def cpu_baseline_adi(args, np, host, dims, seconds):
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
+
+
+def rel_err(np, expected, got) -> float:
+    """normwise relative error ||got - expected|| / ||expected|| in FP64"""
+    e = np.asarray(expected, dtype=np.float64).ravel()
+    g = np.asarray(got, dtype=np.float64).ravel()
+    den = float(np.linalg.norm(e))
+    return float(np.linalg.norm(g - e) / (den if den > 0 else 1.0))
+
+
+def cpu_meta(oracle) -> dict:
+    return {"cores": oracle.num_threads(), "cpu_model": cpu_model(),
+            "omp": f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')} "
+                   f"OMP_PLACES={os.environ.get('OMP_PLACES')}"}
+
+
+def cpu_baseline_adi(np, u0, gpu_uT, n, steps):
+    """The oracle's ADI (C/OpenMP port of apps/adi/src/adi_orig.c, with
+    materialised coefficients as the reference) from the same u0 over the
+    same steps, one call per step on a reused work buffer; value from the
+    best step.  Also the GPU result's error against it."""
     import oracle
-    n = dims[0]
-    u = host[0].ravel().copy()
-    t_total = 0.0
-    steps = 0
-    while t_total < seconds and steps < 20:
+    u = np.ascontiguousarray(u0, dtype=np.float64).ravel().copy()
+    ws = np.empty(4 * u.size, np.float64)
+    times = []
+    for _ in range(steps):
         t0 = time.perf_counter()
-        assert oracle.adi(u, n, n, n, 1, 1.0, 1) == 0
-        t_total += time.perf_counter() - t0
-        steps += 1
-    return dict(value=3 * n ** 3 * steps / t_total / 1e9, unit="Gunknowns/s",
-                cores=oracle.num_threads(), kind="port",
-                sample=f"{steps} ADI time steps of the same {n}^3 workload, oracle C/OpenMP port "
-                       f"of apps/adi/src/adi_orig.c (materialised coefficients, as the reference)")
+        assert oracle.adi(u, n, n, n, 1, 1.0, 1, ws=ws) == 0
+        times.append(time.perf_counter() - t0)
+    best = min(times)
+    return dict(value=3 * n ** 3 / best / 1e9, unit="Gunknowns/s", **cpu_meta(oracle),
+                kind="port",
+                GB_s_9words=round(9 * 8 * n ** 3 / best / 1e9, 2),
+                mean_step_ms=round(1e3 * sum(times) / len(times), 2),
+                best_step_ms=round(1e3 * best, 2),
+                err_vs_fp64_oracle=rel_err(np, u, gpu_uT),
+                sample=f"{steps} ADI time steps of the same {n}^3 workload from the same u0 "
+                       f"(FP64; value from the fastest step, all steps checked), oracle C/OpenMP "
+                       f"port of apps/adi/src/adi_orig.c (materialised coefficients, as the "
+                       f"reference)")
 
 
-def cpu_baseline_plain(args, np, host, dims, seconds):
+def cpu_baseline_plain_step(np, host, dims, gpu_d, reps=5):
+    """The plain headline step on the oracle: the three solves in turn on the
+    same batch (FP64), best of `reps`, and the GPU step's error against it."""
     import oracle
     n = dims[0]
-    # bounded sample: a slab of the same workload (all x-lines of nz_s planes
-    # for solvedim 0 and 1, all z-lines of ny_s rows for solvedim 2)
-    t_total = 0.0
-    unk_total = 0
-    reps = 0
-    while t_total < seconds and reps < 50:
+    h = [np.ascontiguousarray(x, dtype=np.float64) for x in host]
+    times = []
+    for _ in range(reps):
+        d = h[3].copy()
+        t0 = time.perf_counter()
         for sd in (0, 1, 2):
-            a, b, c, d = (x.copy() for x in host)
+            assert oracle.mtsv(h[0], h[1], h[2], d, None, 3, sd, dims) == 0
+        times.append(time.perf_counter() - t0)
+    best = min(times)
+    return dict(value=3 * n ** 3 / best / 1e9, unit="Gunknowns/s", **cpu_meta(oracle),
+                kind="port", GB_s=round(15 * 8 * n ** 3 / best / 1e9, 2),
+                err_vs_fp64_oracle=rel_err(np, d, gpu_d),
+                sample=f"solvedim 0, 1, 2 in turn on the full {n}^3 batch (FP64), oracle "
+                       f"C/OpenMP port of scalar/src/cpu/trid_cpu.cpp, best of {reps}")
+
+
+def cpu_baseline_plain(np, host, dims, gpu_out, reps=5):
+    """Oracle strided solves (C/OpenMP port of scalar/src/cpu/trid_cpu.cpp)
+    of the same 256^3 batch along each solvedim, best of `reps`, and the GPU
+    results' errors against the FP64 oracle."""
+    import oracle
+    n = dims[0]
+    out = dict(unit="Gunknowns/s", **cpu_meta(oracle), kind="port",
+               sample=f"the full {n}^3 batch per solvedim, best of {reps}")
+    h64 = [np.ascontiguousarray(x, dtype=np.float64) for x in host]
+    for sd in (0, 1, 2):
+        times = []
+        for _ in range(reps):
+            d = h64[3].copy()
             t0 = time.perf_counter()
-            assert oracle.mtsv(a, b, c, d, None, 3, sd, dims) == 0
-            t_total += time.perf_counter() - t0
-            unk_total += n ** 3
-        reps += 1
-    return dict(value=unk_total / t_total / 1e9, unit="Gunknowns/s", cores=oracle.num_threads(),
-                kind="port",
-                sample=f"{reps} x (solvedim 0,1,2 on the full {n}^3 grid), oracle C/OpenMP port")
+            assert oracle.mtsv(h64[0], h64[1], h64[2], d, None, 3, sd, dims) == 0
+            times.append(time.perf_counter() - t0)
+        best = min(times)
+        out[f"solvedim{sd}"] = {"ms": round(1e3 * best, 2),
+                                "Gunknowns_s": round(n ** 3 / best / 1e9, 3),
+                                "GB_s": round(5 * 8 * n ** 3 / best / 1e9, 2)}
+        for name, got in gpu_out.get(sd, {}).items():
+            out[f"solvedim{sd}"][f"err_{name}_vs_fp64_oracle"] = rel_err(np, d, got)
+    return out
 
 
 # ---------------------------------------------------------------- workloads
@@ -266,7 +370,7 @@ def _timed(torch, fn, reps):
     return sum(e0.elapsed_time(e1) for e0, e1 in ev) / reps
 
 
-def extra_plain(torch, np, ta, n=256, reps=10):
+def extra_plain(torch, np, ta, cpu, n=256, reps=10):
     lib = ta.lib()
     from tridsolver_amd.api import _iv
     rng = np.random.default_rng(0)
@@ -277,9 +381,11 @@ def extra_plain(torch, np, ta, n=256, reps=10):
     d = rng.uniform(-1.0, 1.0, shape)
     dims = _iv([n, n, n])
     out = {}
+    gpu_out = {0: {}, 1: {}, 2: {}}
     for name, dt, ws, fn in (("f64", np.float64, 8, lib.tridDmtsvStridedBatch),
                              ("f32", np.float32, 4, lib.tridSmtsvStridedBatch)):
         dev = [torch.from_numpy(x.astype(dt)).to("cuda") for x in (a, b, c, d)]
+        d0 = dev[3].clone()
         p = [t.data_ptr() for t in dev]
         for sd in (0, 1, 2):
             def solve():
@@ -294,12 +400,25 @@ def extra_plain(torch, np, ta, n=256, reps=10):
                 "ms": round(ms, 5), "Gunknowns_s": round(n ** 3 / (ms * 1e-3) / 1e9, 2),
                 "GB_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
                 "config": ta.last_config()}
-        del dev
-    return {"workload": f"{n}^3, one solve per solvedim, inputs as the plain workload",
-            "bytes_model": "5 words per unknown (read a, b, c, d; write d)", **out}
+            # one solve from the original right-hand side, for the error check
+            dev[3].copy_(d0)
+            solve()
+            gpu_out[sd][name] = dev[3].cpu().numpy()
+            dev[3].copy_(d0)
+        del dev, d0
+    res = {"workload": f"{n}^3, one solve per solvedim, inputs as the plain workload",
+           "bytes_model": "5 words per unknown (read a, b, c, d; write d)", **out}
+    if cpu:
+        cb = cpu_baseline_plain(np, [a, b, c, d], [n, n, n], gpu_out)
+        for sd in (0, 1, 2):
+            for name in ("f64", "f32"):
+                res[f"{name}_solvedim{sd}"]["err_vs_fp64_oracle"] = \
+                    cb[f"solvedim{sd}"].pop(f"err_{name}_vs_fp64_oracle")
+        res["cpu_baseline"] = cb
+    return res
 
 
-def extra_split(torch, np, ta, reps=10):
+def extra_split(torch, np, ta, cpu, reps=10):
     """Split-line solves: 2^25 unknowns in few, very long systems."""
     lib = ta.lib()
     from tridsolver_amd.api import _iv
@@ -312,11 +431,14 @@ def extra_split(torch, np, ta, reps=10):
     c = rng.uniform(-1.0, -0.1, n)
     b = np.abs(a) + np.abs(c) + rng.uniform(0.5, 1.5, n)
     d = rng.uniform(-1.0, 1.0, n)
+    cases = (("sd0_4Mx8", [1 << 22, 8], 0), ("sd1_64x512K", [64, 1 << 19], 1))
+    gpu_out = {}
     for name, dt, ws, fn in (("f64", np.float64, 8, lib.tridDmtsvStridedBatch),
                              ("f32", np.float32, 4, lib.tridSmtsvStridedBatch)):
         dev = [torch.from_numpy(x.astype(dt)).to("cuda") for x in (a, b, c, d)]
+        d0 = dev[3].clone()
         p = [t.data_ptr() for t in dev]
-        for tag, dims, sd in (("sd0_4Mx8", [1 << 22, 8], 0), ("sd1_64x512K", [64, 1 << 19], 1)):
+        for tag, dims, sd in cases:
             dv = _iv(dims)
 
             def solve():
@@ -332,59 +454,32 @@ def extra_split(torch, np, ta, reps=10):
                 "GB_s_5words": round(gbs, 1), "frac_5words": round(gbs / HBM_PEAK_GBS, 4),
                 "frac_9words": round(gbs * 9 / 5 / HBM_PEAK_GBS, 4),
                 "config": ta.last_config()}
-        del dev
+            dev[3].copy_(d0)
+            solve()
+            gpu_out[(name, tag)] = dev[3].cpu().numpy()
+            dev[3].copy_(d0)
+        del dev, d0
+    if cpu:
+        import oracle
+        cb = dict(unit="Gunknowns/s", **cpu_meta(oracle), kind="port",
+                  sample="the same 2^25 unknowns per layout (FP64), best of 3")
+        for tag, dims, sd in cases:
+            times = []
+            for _ in range(3):
+                dd = d.copy()
+                t0 = time.perf_counter()
+                assert oracle.mtsv(a, b, c, dd, None, 2, sd, dims) == 0
+                times.append(time.perf_counter() - t0)
+            best = min(times)
+            cb[tag] = {"ms": round(1e3 * best, 2), "Gunknowns_s": round(n / best / 1e9, 3),
+                       "GB_s": round(5 * 8 * n / best / 1e9, 2)}
+            for name in ("f64", "f32"):
+                out[f"{name}_{tag}"]["err_vs_fp64_oracle"] = rel_err(np, dd, gpu_out[(name, tag)])
+        out["cpu_baseline"] = cb
     return out
 
 
-def extra_adi_sizes(torch, np, ta, steps=10):
-    """The ADI time step at the other grid sizes and precisions of the ADI
-    configuration: 256^3 FP32, 512^3 FP64 and FP32 (autotuned plans, eager
-    steps and graph replay)."""
-    from tridsolver_amd.adi import AdiPlan
-    out = {"workload": "ADI heat diffusion as the headline (lambda=1, a=c=-lambda, "
-                       "b=1+2lambda, Dirichlet-zero, u0~U(0,1) seed 0), other grids",
-           "bytes_model": "9 words per point per step (rhs: u in, du out; x, y: du in/out; "
-                          "z: du, u in, u out)"}
-    for n, dname in ((256, "f32"), (512, "f64"), (512, "f32")):
-        tdt = torch.float64 if dname == "f64" else torch.float32
-        ws = 8 if dname == "f64" else 4
-        u = torch.zeros((n, n, n), dtype=tdt, device="cuda")
-        g = torch.Generator(device="cuda").manual_seed(0)
-        u[1:-1, 1:-1, 1:-1] = torch.rand((n - 2, n - 2, n - 2), generator=g, device="cuda",
-                                         dtype=tdt)
-        du = torch.empty_like(u)
-        plan = AdiPlan(n, n, n, tdt, 1.0, 1)
-        plan.step(u, du, 3)  # tunes at first use
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        plan.step(u, du, steps, sync=0)
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / steps
-        plan.capture(u, du, steps)
-        plan.replay()
-        torch.cuda.synchronize()
-        e0.record()
-        plan.replay(sync=0)
-        e1.record()
-        torch.cuda.synchronize()
-        gms = e0.elapsed_time(e1) / steps
-        pts = n ** 3
-        out[f"{n}^3_{dname}"] = {
-            "ms_per_step": round(ms, 4), "graph_ms_per_step": round(gms, 4),
-            "Gunknowns_s": round(3 * pts / (ms * 1e-3) / 1e9, 1),
-            "graph_Gunknowns_s": round(3 * pts / (gms * 1e-3) / 1e9, 1),
-            "frac_9words": round(9 * ws * pts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "config": plan.describe()}
-        plan.close()
-        del u, du
-        torch.cuda.empty_cache()
-    return out
-
-
-def extra_ragged(torch, np, ta, reps=10):
+def extra_ragged(torch, np, ta, cpu, reps=10):
     import ctypes as C
     from tridsolver_amd.ragged import RaggedPlan
     lib = ta.lib()
@@ -405,15 +500,19 @@ def extra_ragged(torch, np, ta, reps=10):
            "bytes_model": "5 words per unknown + 8 B per system (offsets)"}
     t_off = torch.from_numpy(off).to("cuda")
     t_idx = torch.from_numpy(idx).to("cuda")
+    gpu_out = {}
     for name, tdt, ws, fn in (("f64", torch.float64, 8, lib.tridDmtsvStridedBatch),
                               ("f32", torch.float32, 4, lib.tridSmtsvStridedBatch)):
         dev = [torch.from_numpy(x).to("cuda").to(tdt) for x in host]
         d0 = dev[3].clone()
+        t0 = time.perf_counter()
         plan = RaggedPlan(t_off, tdt)
+        t_plan = time.perf_counter() - t0
         srt = [x[t_idx] for x in dev]
         sd0 = srt[3].clone()
         # agreement of the two paths on the same systems
         plan.solve(dev[0], dev[1], dev[2], dev[3])
+        gpu_out[name] = dev[3].cpu().numpy()
         p = [x.data_ptr() for x in srt]
         dims = [(C.c_int * 2)(int(L), int(k)) for L, k in zip(uniq, counts)]
         base = [int(s) * ws for s in soff[first]]
@@ -455,20 +554,41 @@ def extra_ragged(torch, np, ta, reps=10):
             "speedup_vs_eager": round(t_eager / t_rag, 2),
             "speedup_vs_graph": None if t_graph is None else round(t_graph / t_rag, 2),
             "max_rel_diff_ragged_vs_per_length": diff,
+            "plan_create_s": round(t_plan, 3),
             "plan": plan.describe()}
         plan.close()
         del dev, srt, g
         torch.cuda.empty_cache()
+    if cpu:
+        import oracle
+        times = []
+        for _ in range(5):
+            dd = host[3].copy()
+            t0 = time.perf_counter()
+            assert oracle.ragged(host[0], host[1], host[2], dd, None, off) == 0
+            times.append(time.perf_counter() - t0)
+        best = min(times)
+        out["cpu_baseline"] = dict(
+            unit="Gunknowns/s", value=round(n / best / 1e9, 3),
+            GB_s=round((5 * 8 * n + 8 * (nsys + 1)) / best / 1e9, 2), ms=round(1e3 * best, 1),
+            **cpu_meta(oracle), kind="port",
+            sample="the same 200,000 systems (FP64), oracle restatement of trid_scalarD per "
+                   "system (scalar/src/cpu/trid_cpu.cpp:597-602), best of 5")
+        for name in ("f64", "f32"):
+            out[name]["err_vs_fp64_oracle"] = rel_err(np, dd, gpu_out[name])
     return out
 
 
-def extra_vadi(torch, np, ta, n=128, steps=16, ckpt=4, reps=5):
+def extra_vadi(torch, np, ta, cpu, n=128, steps=16, ckpt=4, reps=5):
     from tridsolver_amd.autograd import vadi_loss_grad
     lib = ta.lib()
-    kap = torch.from_numpy(fourier_kappa(n, seed=0)).to("cuda")
+    kap_h = fourier_kappa(n, seed=0)
+    kap = torch.from_numpy(kap_h).to("cuda")
     rng = np.random.default_rng(0)
-    u0 = torch.from_numpy(rng.uniform(0, 1, (n, n, n))).to("cuda")
-    tgt = torch.from_numpy(rng.uniform(0, 1, (n, n, n))).to("cuda")
+    u0_h = rng.uniform(0, 1, (n, n, n))
+    tgt_h = rng.uniform(0, 1, (n, n, n))
+    u0 = torch.from_numpy(u0_h).to("cuda")
+    tgt = torch.from_numpy(tgt_h).to("cuda")
     u = u0.clone()
 
     def fwd():
@@ -482,15 +602,35 @@ def extra_vadi(torch, np, ta, n=128, steps=16, ckpt=4, reps=5):
     res = {}
 
     def grad():
-        res["loss"] = vadi_loss_grad(kap, u0, tgt, steps, 1.0, ckpt)[0]
+        res["r"] = vadi_loss_grad(kap, u0, tgt, steps, 1.0, ckpt)
     grad()
     t_grad = _timed(torch, grad, reps)
-    return {"workload": f"variable-diffusivity ADI {n}^3 FP64, {steps} steps, lambda 1, kappa = "
-                        f"exp(0.3 g), g = 8 Fourier modes seed 0, checkpoint every {ckpt} steps",
-            "forward_ms": round(t_fwd, 4), "loss_and_grad_ms": round(t_grad, 4),
-            "grad_over_forward": round(t_grad / t_fwd, 3),
-            "forward_Gunknowns_s": round(3 * n ** 3 * steps / (t_fwd * 1e-3) / 1e9, 2),
-            "loss": res["loss"]}
+    loss, gk, gu = res["r"]
+    out = {"workload": f"variable-diffusivity ADI {n}^3 FP64, {steps} steps, lambda 1, kappa = "
+                       f"exp(0.3 g), g = 8 Fourier modes seed 0, checkpoint every {ckpt} steps",
+           "forward_ms": round(t_fwd, 4), "loss_and_grad_ms": round(t_grad, 4),
+           "grad_over_forward": round(t_grad / t_fwd, 3),
+           "forward_Gunknowns_s": round(3 * n ** 3 * steps / (t_fwd * 1e-3) / 1e9, 2),
+           "loss": loss}
+    if cpu:
+        import oracle
+        gk_h, gu_h = gk.cpu().numpy().ravel(), gu.cpu().numpy().ravel()
+        t0 = time.perf_counter()
+        oracle.vadi_forward(kap_h.ravel(), u0_h.ravel(), n, steps, 1.0)
+        t_cf = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        eloss, egk, egu = oracle.vadi_grad(kap_h.ravel(), u0_h.ravel(), tgt_h.ravel(), n, steps,
+                                           1.0)
+        t_cg = time.perf_counter() - t0
+        out["err_loss_vs_fp64_oracle"] = abs(loss - eloss) / abs(eloss)
+        out["err_grad_kappa_vs_fp64_oracle"] = rel_err(np, egk, gk_h)
+        out["err_grad_u0_vs_fp64_oracle"] = rel_err(np, egu, gu_h)
+        out["cpu_baseline"] = dict(
+            unit="ms", forward_ms=round(1e3 * t_cf, 1), loss_and_grad_ms=round(1e3 * t_cg, 1),
+            **cpu_meta(oracle), kind="port",
+            sample="the same forward pass and loss + gradient (FP64, the oracle's reverse mode "
+                   "without checkpoints), one run each")
+    return out
 
 
 def block_inputs(torch, N, P, M, gen):
@@ -508,16 +648,19 @@ def block_inputs(torch, N, P, M, gen):
     return A, B, Cm, d
 
 
-def extra_block(torch, np, ta, N=128, P=65536, reps=5):
-    from tridsolver_amd.block import btsv, btsv_mixed
+def extra_block(torch, np, ta, cpu, N=128, P=65536, reps=5, sample=2048):
+    from tridsolver_amd.block import btsv, btsv_adjoint, btsv_mixed
     gen = torch.Generator(device="cuda")
     gen.manual_seed(0)
     out = {"workload": f"{P} systems x {N} block rows, FP64, blocks as the block tests",
            "bytes_model": "(3M^2 + 2M) words per block row (read A, B, C, d; write u); "
                           "ref_model adds the C'/d' scratch round trip "
-                          "((5N-3)M^2 + (5N-1)M words per system)"}
+                          "((5N-3)M^2 + (5N-1)M words per system); adjoint: the transposed "
+                          "solve's ref_model plus reading u, g and writing grad_A, grad_B, "
+                          "grad_C, grad_d ((3M^2 + 3M) words per block row)"}
     groups = []
     t_sep = 0.0
+    samples = {}
     for M in (2, 3, 4, 5, 8):
         A, B, Cm, d = block_inputs(torch, N, P, M, gen)
         u = torch.empty_like(d)
@@ -534,12 +677,117 @@ def extra_block(torch, np, ta, N=128, P=65536, reps=5):
                         "frac": round(algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                         "ref_model_GB_s": round(ref / (ms * 1e-3) / 1e9, 1),
                         "ref_model_frac": round(ref / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        if M == 3:
+            # the block adjoint (transposed solve plus the outer-product
+            # gradients) on the same batch
+            g = torch.rand(N, P, M, generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+            btsv_adjoint(A, B, Cm, u, g)
+            t_adj = _timed(torch, lambda: btsv_adjoint(A, B, Cm, u, g, sync=0), reps)
+            adj = ref + (3 * M * M + 3 * M) * N * P * 8
+            out["adjoint_M3"] = {"ms": round(t_adj, 4),
+                                 "ref_model_GB_s": round(adj / (t_adj * 1e-3) / 1e9, 1),
+                                 "ref_model_frac": round(adj / (t_adj * 1e-3) / 1e9
+                                                         / HBM_PEAK_GBS, 4),
+                                 "over_forward": round(t_adj / ms, 3)}
+            del g
+            torch.cuda.empty_cache()
+        if cpu:
+            samples[M] = [x[:, :sample].contiguous().cpu().numpy() for x in (A, B, Cm, d, u)]
         groups.append(dict(A=A, B=B, C=Cm, d=d, Cstar=Cs, dstar=ds, u=u))
     ms = _timed(torch, lambda: btsv_mixed(groups, sync=0), reps)
     out["mixed_all_M_one_launch_ms"] = round(ms, 4)
     out["separate_launches_sum_ms"] = round(t_sep, 4)
+    if cpu:
+        import oracle
+        cb = dict(unit="Gunknowns/s", **cpu_meta(oracle), kind="port",
+                  sample=f"the first {sample} of the {P} systems of each batch (FP64), "
+                         f"oracle block Thomas (block/src/blktrid_cpu.cpp:388-504 with the "
+                         f"index fix), best of 5")
+        for gi, M in enumerate((2, 3, 4, 5, 8)):
+            A_h, B_h, C_h, d_h, u_h = samples[M]
+            times = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                ref_u = oracle.btsv(A_h, B_h, C_h, d_h, N, sample, M, layout=0)
+                times.append(time.perf_counter() - t0)
+            best = min(times)
+            algo = (3 * M * M + 2 * M) * N * sample * 8
+            cb[f"M{M}"] = {"ms": round(1e3 * best, 2),
+                           "Gunknowns_s": round(N * sample * M / best / 1e9, 3),
+                           "GB_s": round(algo / best / 1e9, 2)}
+            out[f"M{M}"]["err_vs_fp64_oracle"] = rel_err(np, ref_u, u_h)
+            mixed_u = groups[gi]["u"][:, :sample].contiguous().cpu().numpy()
+            out[f"M{M}"]["err_mixed_launch_vs_fp64_oracle"] = rel_err(np, ref_u, mixed_u)
+        out["cpu_baseline"] = cb
     del groups
     torch.cuda.empty_cache()
+    return out
+
+
+def extra_adi_sizes(torch, np, ta, cpu, steps=10, check_steps=4):
+    """The ADI time step at the other grid sizes and precisions of the ADI
+    configuration: 256^3 FP32, 512^3 FP64 and FP32 (autotuned plans, eager
+    steps and graph replay).  With the CPU leg, `check_steps` steps from u0
+    are compared with the FP64 oracle's."""
+    from tridsolver_amd.adi import AdiPlan
+    out = {"workload": "ADI heat diffusion as the headline (lambda=1, a=c=-lambda, "
+                       "b=1+2lambda, Dirichlet-zero, u0~U(0,1) seed 0), other grids",
+           "bytes_model": "9 words per point per step (rhs: u in, du out; x, y: du in/out; "
+                          "z: du, u in, u out)"}
+    for n, dname in ((256, "f32"), (512, "f64"), (512, "f32")):
+        tdt = torch.float64 if dname == "f64" else torch.float32
+        ws = 8 if dname == "f64" else 4
+        u = torch.zeros((n, n, n), dtype=tdt, device="cuda")
+        g = torch.Generator(device="cuda").manual_seed(0)
+        u[1:-1, 1:-1, 1:-1] = torch.rand((n - 2, n - 2, n - 2), generator=g, device="cuda",
+                                         dtype=tdt)
+        du = torch.empty_like(u)
+        u0 = u.clone()
+        plan = AdiPlan(n, n, n, tdt, 1.0, 1)
+        plan.step(u, du, 3)  # tunes at first use
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        plan.step(u, du, steps, sync=0)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        plan.capture(u, du, steps)
+        plan.replay()
+        torch.cuda.synchronize()
+        e0.record()
+        plan.replay(sync=0)
+        e1.record()
+        torch.cuda.synchronize()
+        gms = e0.elapsed_time(e1) / steps
+        pts = n ** 3
+        out[f"{n}^3_{dname}"] = {
+            "ms_per_step": round(ms, 4), "graph_ms_per_step": round(gms, 4),
+            "Gunknowns_s": round(3 * pts / (ms * 1e-3) / 1e9, 1),
+            "graph_Gunknowns_s": round(3 * pts / (gms * 1e-3) / 1e9, 1),
+            "frac_9words": round(9 * ws * pts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "config": plan.describe()}
+        if cpu:
+            import oracle
+            u.copy_(u0)
+            plan.step(u, du, check_steps)
+            got = u.cpu().numpy()
+            ref = u0.cpu().numpy().astype(np.float64).ravel()
+            del u0
+            ws_h = np.empty(4 * ref.size, np.float64)
+            t0 = time.perf_counter()
+            assert oracle.adi(ref, n, n, n, check_steps, 1.0, 1, ws=ws_h) == 0
+            t_c = (time.perf_counter() - t0) / check_steps
+            del ws_h
+            out[f"{n}^3_{dname}"]["err_vs_fp64_oracle"] = rel_err(np, ref, got)
+            out[f"{n}^3_{dname}"]["cpu_baseline"] = dict(
+                unit="Gunknowns/s", value=round(3 * pts / t_c / 1e9, 3),
+                ms_per_step=round(1e3 * t_c, 1), **cpu_meta(oracle), kind="port",
+                sample=f"{check_steps} FP64 steps of the same grid from the same u0, mean")
+        plan.close()
+        del u, du
+        torch.cuda.empty_cache()
     return out
 
 
@@ -571,7 +819,9 @@ def main():
         W["step"]()
     torch.cuda.synchronize()
     if "describe" in W:  # the plan tunes at its first use
-        W["kernel_cfg"] = {k: W["describe"]() for k in range(len(W["kernels"]))}
+        W["kernel_cfg"] = {k: stage_config(W["describe"](), W["kernels"][k])
+                           for k in range(len(W["kernels"]))}
+        W["plan"] = W["describe"]()
 
     nk = len(W["kernels"])
     ev = [[[torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)]
@@ -624,16 +874,28 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles(f"{args.workload}|{W['kernels'][dom]}|{args.dtype}"),
+            "traffic": None,
         },
         "kernels": kernels,
         "graph_ms_per_step": (round(W["graph_ms"](args.steps), 5) if "graph_ms" in W else None),
         "wall_s_timed_region": round(wall, 3),
         "gpu_ms_timed_region": round(gpu_ms, 3),
     }
+    dom_cfg = W["kernel_cfg"].get(dom, "")
+    out["roofline"]["config"] = dom_cfg
+    if "plan" in W:
+        out["config"]["plan"] = W["plan"]
+    out["roofline"]["traffic"], out["roofline"]["traffic_source"] = traffic_from_profiles(
+        f"{args.workload}|{W['kernels'][dom]}|{args.dtype}", dom_cfg)
     if not args.no_cpu:
-        fnb = cpu_baseline_adi if args.workload == "adi" else cpu_baseline_plain
-        out["cpu_baseline"] = fnb(args, np, W["host"], W["dims"], args.cpu_seconds)
+        # the GPU result of the same work the oracle runs, then the oracle
+        # (outside every timed region)
+        if args.workload == "adi":
+            gpu_uT = W["result"](args.steps)
+            out["cpu_baseline"] = cpu_baseline_adi(np, W["host"][0], gpu_uT, args.n, args.steps)
+        else:
+            gpu_d = W["result"]()
+            out["cpu_baseline"] = cpu_baseline_plain_step(np, W["host"], W["dims"], gpu_d)
     if not args.no_extras:
         del W
         torch.cuda.empty_cache()
@@ -644,7 +906,7 @@ def main():
             k = k.strip()
             if k:
                 print(f"[bench] extra {k}", file=sys.stderr, flush=True)
-                out["extras"][k] = fns[k](torch, np, ta)
+                out["extras"][k] = fns[k](torch, np, ta, not args.no_cpu)
     print(json.dumps(out), flush=True)
 
 

@@ -13,21 +13,32 @@
  * only the offsets and accesses every element they name.
  *
  * tridRaggedPlanCreate reads the offsets once (device array of nsys+1
- * int64 values, non-decreasing), bins the systems by length into chunked
- * schedules (a power-of-two number of chunks of 5..8 rows, FP32 9..16
- * rows: one lane per system up to 8 rows (FP32 16), 2..64 lanes per system
- * with an in-wave reduced solve, 128..512 lanes per system across the waves
- * of a work-group through LDS) and builds a work list ordered from the most
- * to the least expensive item.  Each solve is one
- * launch of a persistent kernel whose work-groups pull items from an atomic
- * counter (reset by the kernel itself).  FP64 systems of 4097..8192 rows
- * go to a second launch of the same kernel (512 chunks of 9..16 rows).  Systems
- * longer than 8192 rows are solved after it: the longest of them one by
- * one, each as a strided batch
- * of one system (tridXmtsvStridedBatch, split-line solve past 8192 rows),
- * the rest by a one-thread-per-system launch; the plan picks the split by a
- * cost model (a one-system solve ~25 us, the one-thread launch ~0.5 us per
- * row of its longest system).
+ * int64 values, non-decreasing) and bins the systems by length into chunked
+ * schedules: a power-of-two number of chunks per system (1..512 lanes; up to
+ * 64 of them in one wave with an in-wave reduced solve, more across the
+ * waves of a work-group) in one of two row families, chunks of 5..8 rows (up
+ * to 4096 rows per system) or of 9..16 rows (up to 8192).  Each length class
+ * (2^c chunks of the 5..8 family; 4097..8192 rows) takes one family and,
+ * past 64 chunks, one of two reduced solves: PCR through LDS, one barrier per
+ * level, or PCR within each wave followed by a serial sweep across the
+ * waves (three barriers).  Untuned, FP64 takes 5..8 rows up to 4096 rows with
+ * PCR through LDS, FP32 (half the registers per row) 9..16 rows throughout
+ * with the per-wave reduced solve.  With the autotuner on (tridSetAutotune(1)
+ * when the plan is created) the plan times whole solves of a synthetic batch
+ * with its own lengths (a uniform sample of the systems past 2^27 unknowns),
+ * in buffers of its own, and from the untuned choices changes one class at a
+ * time, from the most to the fewest unknowns, keeping a change that makes
+ * the whole solve faster by more than 2 %.
+ * The plan builds a work list per family ordered from the most to the least
+ * expensive item; each solve is one launch of a persistent kernel per
+ * family present, whose work-groups pull items from an atomic counter
+ * (reset by the kernel itself).  Systems longer than 8192 rows are solved
+ * after it: the longest of them one by one, each as a strided batch of one
+ * system (tridXmtsvStridedBatch, split-line solve past 8192 rows), the rest
+ * by a one-thread-per-system launch; the plan picks the split by a cost
+ * model (untuned: a one-system solve ~25 us, the one-thread launch ~0.5 us
+ * per row of its longest system; the autotuner measures both on the plan's
+ * longest system).
  * A plan may be used for any arrays with the same offsets; calls that
  * overlap on one plan race on its counter and must be stream-ordered.
  */
@@ -61,7 +72,17 @@ EXTERN_C tridStatus_t tridSgtsvRaggedBatch(tridRaggedPlan_t plan,
 EXTERN_C tridStatus_t tridRaggedPlanExtent(tridRaggedPlan_t plan,
                                            long long *elements);
 
-/* Summary of the plan's bins ("R8/cps4: 1234 systems, ..."). */
+/* Summary of the plan's bins ("R8/cps4:1234 ..."), each class's choice
+ * ("classes: c3=0w ..." = family 0 rows, per-wave reduced solve; "p" = LDS
+ * PCR) and, for a tuned plan, every candidate's time ("tuned: ..."). */
 EXTERN_C const char *tridRaggedPlanDescribe(tridRaggedPlan_t plan);
+
+/* Diagnostic (no reference counterpart): plans created afterwards put
+ * every length class that the family can hold in row family `family` (0 =
+ * chunks of 5..8 rows, 1 = 9..16; -1 = automatic) and use reduced solve
+ * `red` past 64 chunks (0 = PCR through LDS, 1 = per-wave PCR and a sweep
+ * across waves; -1 = automatic).  Either forced value turns the plan
+ * autotuner off.  For tests of every schedule. */
+EXTERN_C tridStatus_t tridRaggedForceChoice(int family, int red);
 
 #endif

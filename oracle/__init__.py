@@ -59,6 +59,10 @@ def _declare(L: C.CDLL) -> None:
     L.oracle_adi_d.restype = _I
     L.oracle_adi_s.argtypes = [_I, C.c_float, _P, _I, _I, _I, _I]
     L.oracle_adi_s.restype = _I
+    L.oracle_adi_ws_d.argtypes = [_I, C.c_double, _P, _I, _I, _I, _I, _P]
+    L.oracle_adi_ws_d.restype = _I
+    L.oracle_adi_ws_s.argtypes = [_I, C.c_float, _P, _I, _I, _I, _I, _P]
+    L.oracle_adi_ws_s.restype = _I
     L.oracle_adjoint_d.argtypes = [_P] * 9 + [_I, _I, _P, _P]
     L.oracle_adjoint_d.restype = _I
     L.oracle_btsv_d.argtypes = [_P, _P, _P, _P, _P, _I, _I, _I, _I]
@@ -127,7 +131,13 @@ def ragged(a, b, c, d, u, offsets, inc=False) -> int:
              len(offsets) - 1, int(bool(inc)))
 
 
-def adi(u, nx, ny, nz, iters, lam=1.0, mode=0) -> int:
+def adi(u, nx, ny, nz, iters, lam=1.0, mode=0, ws=None) -> int:
+    """ADI time steps on u in place; ws (optional): work buffer of
+    4*nx*ny*nz values of u's dtype, reused across calls."""
+    if ws is not None:
+        assert ws.dtype == u.dtype and ws.size >= 4 * nx * ny * nz
+        f = lib().oracle_adi_ws_d if u.dtype == np.float64 else lib().oracle_adi_ws_s
+        return f(mode, lam, _ptr(u), nx, ny, nz, iters, _ptr(ws))
     if u.dtype == np.float64:
         return lib().oracle_adi_d(mode, lam, _ptr(u), nx, ny, nz, iters)
     return lib().oracle_adi_s(mode, lam, _ptr(u), nx, ny, nz, iters)

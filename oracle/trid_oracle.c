@@ -488,17 +488,19 @@ int oracle_btsv_d(const double *A, const double *B, const double *C,
  * mode 0 is the reference's coefficients a=c=-lambda/2, b=1+lambda
  * (adi_orig.c:189-191); mode 1 is a=c=-lambda, b=1+2 lambda.  The initial
  * condition is the caller's u (the reference sets 1 on the boundary and 0
- * inside, adi_orig.c:297-307).
+ * inside, adi_orig.c:297-307).  oracle_adi_ws_* takes the four work arrays
+ * (du, a, b, c) as one caller buffer of 4*nx*ny*nz values, so that timed
+ * calls do not allocate; oracle_adi_* allocates it per call.
  * ---------------------------------------------------------------------- */
 #define DEFINE_ADI(T, SUF)                                                     \
-  int oracle_adi_##SUF(int mode, T lambda, T *u, int nx, int ny, int nz,       \
-                       int iters) {                                            \
+  int oracle_adi_ws_##SUF(int mode, T lambda, T *u, int nx, int ny, int nz,    \
+                          int iters, T *ws) {                                  \
     long n = (long)nx * ny * nz, ind;                                          \
     int it, i, j, k;                                                           \
     int dims[3] = {nx, ny, nz};                                                \
     T *du, *a, *b, *c;                                                         \
     T ai, bi, ci;                                                              \
-    if (nx < 1 || ny < 1 || nz < 1) return OR_INVALID_VALUE;                   \
+    if (nx < 1 || ny < 1 || nz < 1 || !ws) return OR_INVALID_VALUE;            \
     if (mode == 0) {                                                           \
       ai = (T)-0.5f * lambda;                                                  \
       bi = (T)1.0f + lambda;                                                   \
@@ -508,14 +510,10 @@ int oracle_btsv_d(const double *A, const double *B, const double *C,
       bi = (T)1.0f + (T)2.0f * lambda;                                         \
       ci = -lambda;                                                            \
     }                                                                          \
-    du = (T *)malloc(sizeof(T) * (size_t)n);                                   \
-    a = (T *)malloc(sizeof(T) * (size_t)n);                                    \
-    b = (T *)malloc(sizeof(T) * (size_t)n);                                    \
-    c = (T *)malloc(sizeof(T) * (size_t)n);                                    \
-    if (!du || !a || !b || !c) {                                               \
-      free(du); free(a); free(b); free(c);                                     \
-      return OR_ALLOC_FAILED;                                                  \
-    }                                                                          \
+    du = ws;                                                                   \
+    a = ws + n;                                                                \
+    b = ws + 2 * n;                                                            \
+    c = ws + 3 * n;                                                            \
     for (it = 0; it < iters; it++) {                                           \
       _Pragma("omp parallel for collapse(2) private(i, ind)")                  \
       for (k = 0; k < nz; k++)                                                 \
@@ -544,8 +542,19 @@ int oracle_btsv_d(const double *A, const double *B, const double *C,
       oracle_mtsv_##SUF(a, dims, b, dims, c, dims, du, dims, u, dims, 3, 2,    \
                         dims, 1);                                              \
     }                                                                          \
-    free(du); free(a); free(b); free(c);                                       \
     return OR_SUCCESS;                                                         \
+  }                                                                            \
+  int oracle_adi_##SUF(int mode, T lambda, T *u, int nx, int ny, int nz,       \
+                       int iters) {                                            \
+    long n = (long)nx * ny * nz;                                               \
+    T *ws;                                                                     \
+    int st;                                                                    \
+    if (nx < 1 || ny < 1 || nz < 1) return OR_INVALID_VALUE;                   \
+    ws = (T *)malloc(sizeof(T) * 4 * (size_t)n);                               \
+    if (!ws) return OR_ALLOC_FAILED;                                           \
+    st = oracle_adi_ws_##SUF(mode, lambda, u, nx, ny, nz, iters, ws);          \
+    free(ws);                                                                  \
+    return st;                                                                 \
   }
 DEFINE_ADI(double, d)
 DEFINE_ADI(float, s)

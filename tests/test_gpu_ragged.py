@@ -128,6 +128,65 @@ def test_ragged_long_systems_both_routes(dtype, inc):
     check_close(exp_u if inc else exp_d, got, dtype, f"ragged [{desc}]")
 
 
+@pytest.mark.parametrize("fam,red", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_ragged_forced_choices(fam, red, dtype):
+    """Every schedule the autotuner chooses from: both row families (5..8
+    and 9..16 rows per chunk) and both reduced solves past 64 chunks (PCR
+    through LDS, per-wave PCR with a sweep across waves), at the edges of
+    every chunk count."""
+    from tridsolver_amd.ragged import force_choice
+    rng = np.random.default_rng(20 + fam * 2 + red)
+    edges = [L for cl in range(10) for L in ((8 << cl) - 1, 8 << cl, (8 << cl) + 1)]
+    lengths = np.array(EDGE + edges + list(rng.integers(1, 8193, 60)), dtype=np.int64)
+    lengths = lengths[lengths <= 8192]
+    rng.shuffle(lengths)
+    off, arrs = ragged_inputs(lengths, dtype, seed=6)
+    force_choice(fam, red)
+    try:
+        (got,), plan = run(off, arrs, dtype)
+    finally:
+        force_choice(-1, -1)
+    desc = plan.describe()
+    for c in range(10):
+        assert f"c{c}={fam}{'w' if red else 'p'}" in desc, desc
+    assert f"c10=1{'w' if red else 'p'}" in desc, desc
+    exp = arrs[3].copy()
+    assert oracle.ragged(arrs[0], arrs[1], arrs[2], exp, None, off) == 0
+    check_close(exp, got, dtype, f"ragged [{desc}]")
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_ragged_tuned_plan(dtype):
+    """A plan created with the autotuner on times every class's candidates
+    (and the long-system cost constants) and still solves every system as
+    the oracle does, with and without Inc."""
+    import tridsolver_amd as ta
+    rng = np.random.default_rng(31)
+    lengths = np.array([30000, 9000] + list(rng.integers(8193, 8400, 5)) +
+                       [int(x) for x in np.exp(rng.uniform(0, np.log(8192), 3000))],
+                       dtype=np.int64)
+    rng.shuffle(lengths)
+    off, arrs = ragged_inputs(lengths, dtype, seed=8)
+    u = rng.uniform(-1, 1, int(off[-1])).astype(dtype)
+    ta.set_autotune(1)
+    try:
+        (got,), plan = run(off, arrs, dtype)
+        (got_u,), _ = run(off, arrs, dtype, inc=True, u=u)
+    finally:
+        ta.set_autotune(0)
+    desc = plan.describe()
+    assert "tuned:" in desc and "long:solo=" in desc, desc
+    print(f"tuned ragged plan {np.dtype(dtype).name}: {desc}")
+    exp = arrs[3].copy()
+    assert oracle.ragged(arrs[0], arrs[1], arrs[2], exp, None, off) == 0
+    check_close(exp, got, dtype, f"ragged [{desc}]")
+    exp_d = arrs[3].copy()
+    exp_u = u.copy()
+    assert oracle.ragged(arrs[0], arrs[1], arrs[2], exp_d, exp_u, off, inc=True) == 0
+    check_close(exp_u, got_u, dtype, f"ragged inc [{desc}]")
+
+
 def test_ragged_repeated_solves_one_plan():
     lengths = np.array([5, 300, 1200, 17, 4000, 64, 9000] * 40, dtype=np.int64)
     off, arrs = ragged_inputs(lengths, np.float64, seed=2)
@@ -159,6 +218,6 @@ def test_ragged_benchmark_configuration(dtype):
     (got,), plan = run(off, arrs, dtype)
     exp = arrs[3].copy()
     assert oracle.ragged(arrs[0], arrs[1], arrs[2], exp, None, off) == 0
-    r = check_close(exp, got, dtype, "ragged benchmark")
+    r = check_close(exp, got, dtype, f"ragged benchmark [{plan.describe()}]")
     print(f"ragged 200k systems, {int(off[-1])} unknowns, {np.dtype(dtype).name}: rel err {r:.2e} "
           f"[{plan.describe()}]")

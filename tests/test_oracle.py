@@ -74,6 +74,15 @@ def test_oracle_adi_bitexact_vs_reference_gold(name):
     u = u.ravel().copy()
     assert oracle.adi(u, nx, ny, nz, it, 1.0, 0) == 0
     assert np.array_equal(u, ref), f"max diff {np.max(np.abs(u - ref))}"
+    # the caller-workspace form, one step per call on a reused (dirty)
+    # buffer, reproduces the gold run bit for bit
+    u2 = np.zeros((nz, ny, nx), dt)
+    u2[0], u2[-1], u2[:, 0], u2[:, -1], u2[:, :, 0], u2[:, :, -1] = 1, 1, 1, 1, 1, 1
+    u2 = u2.ravel().copy()
+    ws = np.full(4 * u2.size, np.nan, dt)
+    for _ in range(it):
+        assert oracle.adi(u2, nx, ny, nz, 1, 1.0, 0, ws=ws) == 0
+    assert np.array_equal(u2, ref)
 
 
 # ---------------------------------------------------------------------------

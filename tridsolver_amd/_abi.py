@@ -54,6 +54,7 @@ SIGNATURES = {
     "tridDgtsvRaggedBatch": (ST, [P, P, P, P, P, P, I, I]),
     "tridSgtsvRaggedBatch": (ST, [P, P, P, P, P, P, I, I]),
     "tridRaggedPlanDescribe": (C.c_char_p, [P]),
+    "tridRaggedForceChoice": (ST, [I, I]),
     "tridRaggedPlanExtent": (ST, [P, C.POINTER(C.c_longlong)]),
     # trid_adjoint.h
     "tridDmtsvStridedBatchAdjoint": (ST, [P] * 9 + [I, I, IP, IP, I]),

@@ -258,19 +258,26 @@ __device__ __forceinline__ void blk2_problem(const BlkArgs<T> &g, long long p, b
     Cp[i] = T(0);
     dp[i] = T(0);
   }
-  for (int n = 0; n < N; n++) {
-    // the row's tiles: sub-diagonal (A_n; C_{n-1} transposed), diagonal
-    // (B_n; transposed), super-diagonal (C_n; A_{n+1} transposed)
-    const bool has_sub = n > 0, has_sup = n < N - 1;
-    T ra[M], rb[M], rc[M], rd;
+  // the row's tiles: sub-diagonal (A_n; C_{n-1} transposed), diagonal
+  // (B_n; transposed), super-diagonal (C_n; A_{n+1} transposed), this
+  // lane's elements j, j+M, ...  Row n+1 is loaded while row n is
+  // eliminated: with at most a few problems per SIMD the recurrence would
+  // otherwise wait a full memory latency every block row.
+  T ra[M], rb[M], rc[M], rd;
+  auto load_row = [&](int n) {
+    const bool sub_n = n > 0, sup_n = n < N - 1;
 #pragma unroll
     for (int r = 0; r < M; r++) {
       const int e = j + M * r;
-      ra[r] = has_sub ? (TRANS ? g.C[blk(n - 1) + e] : g.A[blk(n) + e]) : T(0);
+      ra[r] = sub_n ? (TRANS ? g.C[blk(n - 1) + e] : g.A[blk(n) + e]) : T(0);
       rb[r] = g.B[blk(n) + e];
-      rc[r] = has_sup ? (TRANS ? g.A[blk(n + 1) + e] : g.C[blk(n) + e]) : T(0);
+      rc[r] = sup_n ? (TRANS ? g.A[blk(n + 1) + e] : g.C[blk(n) + e]) : T(0);
     }
     rd = g.d[vec(n) + j];
+  };
+  load_row(0);
+  for (int n = 0; n < N; n++) {
+    const bool has_sub = n > 0, has_sup = n < N - 1;
     if (lane_ok) {
 #pragma unroll
       for (int r = 0; r < M; r++) {
@@ -282,6 +289,7 @@ __device__ __forceinline__ void blk2_problem(const BlkArgs<T> &g, long long p, b
       sd[j] = rd;
     }
     wb();
+    if (n + 1 < N) load_row(n + 1);
     // element (i, c) of the row's sub/diag/sup block
     auto at_t = [&](const T *t, int i, int c) { return TRANS ? t[c * M + i] : t[i * M + c]; };
     T X[M], Cn[M], v[M];
@@ -356,18 +364,24 @@ __device__ __forceinline__ void blk2_problem(const BlkArgs<T> &g, long long p, b
 #pragma unroll
   for (int i = 0; i < M; i++) un[i] = dp[i];
   if (store) g.u[vec(N - 1) + j] = un[j];
-  for (int n = N - 2; n >= 0; n--) {
-    // C'_n and d'_n as this lane's group stored them (stores by the other
-    // lanes of a wave are visible to its later loads)
-    T rc[M];
+  // C'_n and d'_n as this lane's group stored them (stores by the other
+  // lanes of a wave are visible to its later loads); row n-1 is loaded
+  // while row n is substituted
+  T bc[M], bd = T(0);
+  auto load_back = [&](int n) {
 #pragma unroll
-    for (int r = 0; r < M; r++) rc[r] = g.Cs[blk(n) + j + M * r];
-    const T dj = g.ds[vec(n) + j];
+    for (int r = 0; r < M; r++) bc[r] = g.Cs[blk(n) + j + M * r];
+    bd = g.ds[vec(n) + j];
+  };
+  if (N >= 2) load_back(N - 2);
+  for (int n = N - 2; n >= 0; n--) {
+    const T dj = bd;
     if (lane_ok) {
 #pragma unroll
-      for (int r = 0; r < M; r++) sC[j + M * r] = rc[r];
+      for (int r = 0; r < M; r++) sC[j + M * r] = bc[r];
     }
     wb();
+    if (n >= 1) load_back(n - 1);
     T acc = dj;
 #pragma unroll
     for (int k = 0; k < M; k++) acc -= sC[j * M + k] * un[k];
@@ -397,10 +411,12 @@ __device__ __forceinline__ void blk2_block(const BlkArgs<T> &g, long long block,
 }
 
 // Minimum waves per SIMD for the register allocator (the live set is about
-// 5M values per lane; without a bound the scheduler hoists tile loads and
-// doubles it).
+// 5M values per lane plus the next row's 3M + 1; without a bound the
+// scheduler hoists tile loads and doubles it).  65,536 problems fill about
+// 3 (M = 3) to 6 (M = 6) waves per SIMD of MI355X, so the bound costs no
+// occupancy that a batch of that size would use.
 constexpr int blk2_min_waves(int M) {
-  return M <= 2 ? 8 : (M == 3 ? 5 : (M == 4 ? 4 : (M == 5 ? 3 : 2)));
+  return M <= 2 ? 8 : (M == 3 ? 4 : (M == 4 ? 3 : 2));
 }
 
 template <typename T, int M, int TRANS>

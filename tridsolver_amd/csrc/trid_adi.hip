@@ -483,10 +483,7 @@ int pick_in_step(int ncand, Prefix prefix, Cand cand, Suffix suffix, hipStream_t
   for (auto &e : ev) (void)hipEventDestroy(e);
   (void)hipGetLastError();
   if (times) *times = best;
-  int arg = -1;
-  for (int i = 0; i < ncand; i++)
-    if (best[i] < 1e29f && (arg < 0 || best[i] < best[arg])) arg = i;
-  return arg;
+  return stable_pick(best);
 }
 
 // TRIDHIP_TUNE_LOG=1: the plan tuner prints every candidate's time.

@@ -368,6 +368,18 @@ inline PartCfg pick_rows(const PartCfg &c, int N, bool gvec, bool dense, size_t 
 // round candidate i gets prep(i) (untimed), one warm-up run(i) and `reps`
 // timed run(i) launches on st.  Returns the index with the lowest time of
 // its best round (-1 if no candidate could be launched).
+// The first candidate (in the caller's order) within `tol` of the fastest
+// time: a choice that timing noise between near-equal candidates does not
+// flip from run to run.  -1 when no candidate ran.
+inline int stable_pick(const std::vector<float> &t, float tol = 0.02f) {
+  float lo = 1e30f;
+  for (float v : t) lo = std::min(lo, v);
+  if (!(lo < 1e29f)) return -1;
+  for (size_t i = 0; i < t.size(); i++)
+    if (t[i] <= lo * (1.0f + tol)) return (int)i;
+  return -1;
+}
+
 template <typename Prep, typename Run>
 int pick_fastest(int ncand, Prep prep, Run run, hipStream_t st,
                  std::vector<float> *times = nullptr, int reps = 5, int passes = 2) {
@@ -398,10 +410,7 @@ int pick_fastest(int ncand, Prep prep, Run run, hipStream_t st,
   (void)hipEventDestroy(e1);
   (void)hipGetLastError();
   if (times) *times = best;
-  int arg = -1;
-  for (int i = 0; i < ncand; i++)
-    if (best[i] < 1e29f && (arg < 0 || best[i] < best[arg])) arg = i;
-  return arg;
+  return stable_pick(best);
 }
 
 // Split-line configuration for systems longer than one work-group holds:

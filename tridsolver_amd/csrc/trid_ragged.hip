@@ -2,59 +2,67 @@
 // (include/trid_ragged.h).
 //
 // Every system is solved with the partitioned Thomas scheme of
-// trid_part.hpp (chunks of R = 8 rows per lane, reduced system by PCR), with the number of chunks per system (cps,
-// 1..512) chosen per length bin.  A work item is one work-group of 512
-// threads processing 512/cps systems of one bin.
-// Loads are staged per span group through LDS: the lanes of a system (or of
-// one wave's 64 chunks of a longer system) read its contiguous span as
-// 16-byte pieces, then each lane takes its chunk from LDS (the transpose of
-// the solvedim-0 path, for spans at arbitrary offsets and alignments).
+// trid_part.hpp: chunks of R rows per lane (Thomas within the chunk), the
+// chunks' reduced system solved across lanes.  The chunk length R and the
+// number of chunks per system (cps, 1..512) are chosen per length class.  A
+// work item is one work-group of 512 threads processing 512/cps systems of
+// one bin.  Loads are staged per span group through LDS: the lanes of a
+// system (or of one wave's 64 chunks of a longer system) read its contiguous
+// span as 16-byte pieces, then each lane takes its chunk from LDS (the
+// transpose of the solvedim-0 path, for spans at arbitrary offsets and
+// alignments).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/trid_hip.h"
 #include "../../include/trid_hip_ext.h"
 #include "../../include/trid_ragged.h"
+#include "trid_launch.hpp"
 #include "trid_part.hpp"
 #include "trid_runtime.hpp"
 
 namespace tridhip {
 
 constexpr int kRagThreads = 512;
-constexpr int kRagMaxLen = 4096;  // longest system of the persistent kernel
-// systems of kRagMaxLen+1 .. kRagLongLen rows: a second launch of the
-// persistent kernel with 512 chunks of 9..16 rows (more registers per lane,
-// so it does not share the first launch's instantiation)
-constexpr int kRagLongLen = 8192;
-constexpr int kRagR2min = 9, kRagR2max = 16;
-// cost model of the systems past kRagLongLen rows, measured on MI355X (FP64):
-// one strided one-system solve takes ~kSoloUs (27 us for 9000 rows, 42 us
-// for 2^20 rows: launches and latency); the one-thread kernel ~kThreadRowUs
-// per row of its longest system (0.4-0.8 us: a dependent chain of global
-// accesses)
+// Two row families, each a launch of the persistent kernel with its own
+// instantiation (registers are those of the family's longest chunk):
+//   family 0: chunks of 5..8 rows   (512 chunks cover 4096 rows)
+//   family 1: chunks of 9..16 rows  (512 chunks cover 8192 rows)
+// A bin is (cps, R): cps a power of two (1..512) and R the shortest row count
+// of the family with cps * R >= L, so that at least 4/5 (family 1: 8/9) of a
+// system's lanes hold rows.
+constexpr int kRagRlo[2] = {5, 9}, kRagRhi[2] = {8, 16};
+constexpr int kRagMaxLen[2] = {kRagThreads * 8, kRagThreads * 16};
+// systems past kRagMaxLen[1] rows leave the persistent kernel: the longest
+// of them are solved one by one as strided batches of one system, the rest
+// by a one-thread-per-system kernel.  Untuned cost model, measured on
+// MI355X (FP64): one strided one-system solve ~kSoloUs (27 us for 9000 rows,
+// 42 us for 2^20 rows: launches and latency); the one-thread kernel
+// ~kThreadRowUs per row of its longest system (a dependent chain of global
+// accesses).  The autotuner measures both on the plan's own lengths.
 constexpr double kSoloUs = 25.0;
 constexpr double kThreadRowUs = 0.5;
-
-// Chunk lengths of the persistent kernel.  A bin is (cps, R): cps a power of
-// two (1..512) and R the shortest of 5..8 rows (FP64) with cps * R >= L, so
-// that at least 4/5 of a system's lanes hold rows (a power-of-two chunk count
-// with one fixed R leaves a quarter idle on average).  The kernel's registers
-// are those of the longest, 8 rows (512 chunks of 8 rows cover 4096 rows).
-// FP32 rows take half the registers: its bins use 9..16 rows (kRagR2min..
-// kRagR2max), half the chunks and reduced-system levels, up to 8192 rows in
-// the first launch.
-constexpr int kRagRmin = 5, kRagRmax = 8;
+// length classes: class c < 10 holds the systems whose family-0 bin has
+// 2^c chunks (8 * 2^(c-1) < L <= 8 * 2^c); class 10 holds 4097..8192 rows,
+// which only family 1 takes
+constexpr int kRagClasses = 11;
 
 struct RagItem {
   int cps_log2;  // chunks per system
   int first;     // first entry of the item in the sorted system list
   int count;     // systems in this item (<= kRagThreads / cps)
   int rows;      // chunk length R of the item's bin (5..8, or 9..16)
+  int red;       // reduced solve past 64 chunks: 0 = PCR through LDS (one
+                 // barrier per level), 1 = PCR in each wave, then a serial
+                 // sweep across the waves (three barriers)
+  int pad_;
 };
 
 template <typename T>
@@ -203,10 +211,14 @@ __device__ __forceinline__ void rag_item(const RagArgs<T> &g, const RagItem &it,
   if (cps <= kWave) {
     reduced_wave<T>(f, l, k, cps, tid & (kWave - 1), xf, xl);
   } else {
-    // the PCR exchange buffers (6 values per thread) reuse the staging
-    // region once every wave is done with it
+    // the reduced solve's LDS (PCR: 6 values per thread; per wave: 12
+    // values per wave of a system) reuses the staging region once every
+    // wave is done with it
     __syncthreads();
-    reduced_lds<T>(f, l, k, cps, slot, kRagThreads >> it.cps_log2, smem, xf, xl);
+    if (it.red)
+      reduced_multi<T>(f, l, k, cps, slot, smem, xf, xl);
+    else
+      reduced_lds<T>(f, l, k, cps, slot, kRagThreads >> it.cps_log2, smem, xf, xl);
     __syncthreads();
   }
   T y[R];
@@ -261,7 +273,7 @@ __global__ void __launch_bounds__(kRagThreads) k_ragged(RagArgs<T> g) {
   }
 }
 
-// Systems longer than kRagMaxLen: one thread per system, forward sweep's
+// Systems longer than kRagMaxLen[1]: one thread per system, forward sweep's
 // c'/d' in a workspace (ws_off[j] = start of system j's 2*L slots).
 template <typename T>
 __global__ void __launch_bounds__(64)
@@ -296,26 +308,47 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// Synthetic diagonally dominant rows for the autotuner's timing runs.
+template <typename T>
+__global__ void k_rag_fill(T *a, T *b, T *c, T *d, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  a[i] = T(-0.25);
+  b[i] = T(1);
+  c[i] = T(-0.25);
+  d[i] = T((i % 13) - 6) / T(8);
+}
+
 }  // namespace tridhip
 
 using namespace tridhip;
+
+namespace {
+
+// One launch of the persistent kernel: the items of one row family.
+struct RagLaunch {
+  int *sys = nullptr;
+  RagItem *items = nullptr;
+  int nitems = 0;
+  int grid = 0;
+  size_t lds = 0;
+};
+
+// Row family and reduced solve of one length class.
+struct RagChoice {
+  int fam = 0;
+  int red = 1;
+};
+
+}  // namespace
 
 struct tridRaggedPlan_st {
   int dbl = 1;
   long long nsys = 0;
   long long extent = 0;  // offsets[nsys]: elements every array must hold
   const int64_t *offsets = nullptr;  // caller's device array
-  int *sys = nullptr;
-  RagItem *items = nullptr;
-  int nitems = 0;
-  unsigned *counter = nullptr;
-  int grid = 0;
-  // second launch: systems of kRagMaxLen+1 .. kRagLongLen rows
-  int *sys2 = nullptr;
-  RagItem *items2 = nullptr;
-  int nitems2 = 0;
-  int grid2 = 0;
-  size_t lds2 = 0;
+  unsigned *counter = nullptr;  // both launches' (each resets it at its end)
+  RagLaunch launch[2];          // family 0, family 1
   int *long_ids = nullptr;
   long long *long_ws_off = nullptr;
   int nlong = 0;
@@ -323,22 +356,404 @@ struct tridRaggedPlan_st {
   // the longest systems, each solved as a one-system strided batch (split-line
   // solve past 8192 rows): (offset, length)
   std::vector<std::pair<long long, int>> solo;
-  size_t lds = 0;
   std::string text;
 };
 
 namespace {
 
+std::mutex g_rag_mu;
+RagChoice g_rag_forced{-1, -1};  // tridRaggedForceChoice
+
+void free_launch(RagLaunch &l) {
+  (void)hipFree(l.sys);
+  (void)hipFree(l.items);
+  l = RagLaunch{};
+}
+
 void free_plan(tridRaggedPlan_st *p) {
   if (!p) return;
-  (void)hipFree(p->sys);
-  (void)hipFree(p->items);
-  (void)hipFree(p->sys2);
-  (void)hipFree(p->items2);
+  free_launch(p->launch[0]);
+  free_launch(p->launch[1]);
   (void)hipFree(p->counter);
   (void)hipFree(p->long_ids);
   (void)hipFree(p->long_ws_off);
   delete p;
+}
+
+int ceil_log2(long long v) {
+  int l = 0;
+  while ((1LL << l) < v) l++;
+  return l;
+}
+
+// Bin of a system of L rows in family f: cps = 2^cl, R rows per chunk.
+int bin_of(long long L, int f, int &R) {
+  const int cl = ceil_log2((L + kRagRhi[f] - 1) / kRagRhi[f]);
+  R = (int)std::max<long long>(kRagRlo[f], (L + (1LL << cl) - 1) >> cl);
+  return cl;
+}
+
+int class_of(long long L) {
+  if (L > kRagMaxLen[0]) return 10;
+  int R;
+  return bin_of(L, 0, R);
+}
+
+bool fam_ok(int c, int f) { return f == 1 || c < 10; }
+
+// The reduced-solve choice matters only past 64 chunks per system.
+bool red_matters(int c, int f) {
+  const long long Lmax = c < 10 ? 8LL << c : kRagMaxLen[1];
+  int R;
+  return bin_of(Lmax, f, R) > 6;
+}
+
+RagChoice default_choice(int c, bool dbl) {
+  // measured on MI355X over 200,000 systems of 8..4096 rows: FP64 runs
+  // fastest with family 0 up to 4096 rows (family 1 past it) and PCR
+  // through LDS (1.26 ms against 1.31-1.35 ms for the other three
+  // combinations); FP32 (half the registers per row) with family 1
+  // throughout and the per-wave reduced solve (0.68 ms against 0.69-0.78)
+  RagChoice ch;
+  ch.fam = dbl && c < 10 ? 0 : 1;
+  ch.red = dbl ? 0 : 1;
+  return ch;
+}
+
+// Systems of each row family and the reduced solve of each chunk count,
+// from the per-class choices.
+void family_lists(const std::vector<std::vector<int>> &cls, const RagChoice *choice, int f,
+                  std::vector<int> &ids, int *red) {
+  ids.clear();
+  for (int i = 0; i < 10; i++) red[i] = 0;
+  for (int c = 0; c < kRagClasses; c++) {
+    if (choice[c].fam != f) continue;
+    ids.insert(ids.end(), cls[c].begin(), cls[c].end());
+    // the items of 2^cl chunks take the reduced solve of the class they come
+    // from (in family 1, a class's systems take half its chunk count)
+    const long long Lmax = c < 10 ? 8LL << c : kRagMaxLen[1];
+    int R;
+    const int cl = bin_of(Lmax, f, R);
+    if (cl > 6) red[cl] = choice[c].red;
+  }
+  std::sort(ids.begin(), ids.end());
+}
+
+// Work items of the systems `ids` (lengths from `off`) in family f, appended
+// to sys/items, most expensive first (more chunks, then longer chunks).
+// red[cl]: reduced solve of the items with 2^cl chunks.
+void build_items(const std::vector<int64_t> &off, const std::vector<int> &ids, int f,
+                 const int *red, std::vector<int> &sys, std::vector<RagItem> &items,
+                 std::string *desc) {
+  const int nR = kRagRhi[f] - kRagRlo[f] + 1;
+  std::vector<std::vector<int>> bins(10 * nR);
+  for (int id : ids) {
+    int R;
+    const int cl = bin_of(off[id + 1] - off[id], f, R);
+    bins[cl * nR + (R - kRagRlo[f])].push_back(id);
+  }
+  for (int cl = 9; cl >= 0; cl--)
+    for (int R = kRagRhi[f]; R >= kRagRlo[f]; R--) {
+      const std::vector<int> &v = bins[cl * nR + (R - kRagRlo[f])];
+      if (v.empty()) continue;
+      const int per = std::max(1, kRagThreads >> cl);
+      for (size_t i = 0; i < v.size(); i += per) {
+        RagItem it;
+        it.cps_log2 = cl;
+        it.first = (int)(sys.size() + i);
+        it.count = (int)std::min<size_t>(per, v.size() - i);
+        it.rows = R;
+        it.red = cl > 6 ? red[cl] : 0;
+        it.pad_ = 0;
+        items.push_back(it);
+      }
+      sys.insert(sys.end(), v.begin(), v.end());
+      if (desc) {
+        char buf[96];
+        snprintf(buf, sizeof buf, "R%d/cps%d:%zu ", R, 1 << cl, v.size());
+        *desc += buf;
+      }
+    }
+}
+
+template <typename T, int RLO, int RHI>
+void occupancy(size_t lds, int &per_cu) {
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ragged<T, RLO, RHI>,
+                                                      kRagThreads, lds);
+  per_cu = std::max(1, per_cu);
+}
+
+size_t family_lds(int f, size_t ts) {
+  // span-group regions (at most 512 groups of G*R + 2E elements, G >= 1);
+  // the reduced solves reuse them
+  return (size_t)kRagThreads * (kRagRhi[f] + 2 * (16 / ts)) * ts;
+}
+
+int family_grid(int f, bool dbl, int nitems) {
+  int dev = 0, cus = 256, per_cu = 1;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const size_t lds = family_lds(f, dbl ? 8 : 4);
+  if (dbl) {
+    if (f == 0) occupancy<double, 5, 8>(lds, per_cu);
+    else occupancy<double, 9, 16>(lds, per_cu);
+  } else {
+    if (f == 0) occupancy<float, 5, 8>(lds, per_cu);
+    else occupancy<float, 9, 16>(lds, per_cu);
+  }
+  return std::max(1, std::min(nitems, cus * per_cu));
+}
+
+template <typename T>
+hipError_t launch_family(int f, const RagLaunch &l, RagArgs<T> g, hipStream_t st) {
+  if (l.nitems == 0) return hipSuccess;
+  g.sys = l.sys;
+  g.items = l.items;
+  g.nitems = l.nitems;
+  if (f == 0)
+    hipLaunchKernelGGL((k_ragged<T, 5, 8>), dim3(l.grid), dim3(kRagThreads), l.lds, st, g);
+  else
+    hipLaunchKernelGGL((k_ragged<T, 9, 16>), dim3(l.grid), dim3(kRagThreads), l.lds, st, g);
+  return hipGetLastError();
+}
+
+tridStatus_t upload(void **dst, const void *src, size_t bytes) {
+  if (bytes == 0) return TRID_STATUS_SUCCESS;
+  tridStatus_t r = check_hip(hipMalloc(dst, bytes), "hipMalloc");
+  if (r) return r;
+  return check_hip(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+}
+
+tridStatus_t make_launch(RagLaunch &l, int f, bool dbl, const std::vector<int> &sys,
+                         const std::vector<RagItem> &items) {
+  l.nitems = (int)items.size();
+  l.lds = family_lds(f, dbl ? 8 : 4);
+  l.grid = family_grid(f, dbl, l.nitems);
+  tridStatus_t s = upload((void **)&l.sys, sys.data(), sys.size() * sizeof(int));
+  if (!s) s = upload((void **)&l.items, items.data(), items.size() * sizeof(RagItem));
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner
+//
+// For every length class present, the candidate choices (row family, and
+// past 64 chunks per system the reduced solve) are timed on a synthetic
+// batch of the class's own lengths (cycled until it holds 2^22 unknowns, so
+// that the persistent grid runs several rounds of items) and the fastest is
+// kept.  For systems past 8192 rows the two cost constants of the
+// solo / one-thread split are measured on the plan's longest system.  The
+// timing runs use buffers of their own; the caller's arrays are not touched.
+
+struct TuneBufs {
+  void *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr;
+  int64_t *off = nullptr;
+  unsigned *counter = nullptr;
+  ~TuneBufs() {
+    (void)hipFree(a);
+    (void)hipFree(b);
+    (void)hipFree(c);
+    (void)hipFree(d);
+    (void)hipFree(off);
+    (void)hipFree(counter);
+  }
+};
+
+template <typename T>
+tridStatus_t alloc_synthetic(TuneBufs &tb, long long n, hipStream_t st) {
+  const size_t bytes = (size_t)std::max(1LL, n) * sizeof(T);
+  tridStatus_t s = check_hip(hipMalloc(&tb.a, bytes), "hipMalloc");
+  if (!s) s = check_hip(hipMalloc(&tb.b, bytes), "hipMalloc");
+  if (!s) s = check_hip(hipMalloc(&tb.c, bytes), "hipMalloc");
+  if (!s) s = check_hip(hipMalloc(&tb.d, bytes), "hipMalloc");
+  if (s) return s;
+  hipLaunchKernelGGL(k_rag_fill<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     static_cast<T *>(tb.a), static_cast<T *>(tb.b), static_cast<T *>(tb.c),
+                     static_cast<T *>(tb.d), n);
+  return check_hip(hipGetLastError(), "fill");
+}
+
+// Time of one solve with both family launches (best of `passes` x `reps`
+// launches after a warm-up), in ms; < 0 on a launch failure.
+template <typename T>
+float time_launches(const RagLaunch *ls, const RagArgs<T> &g, hipStream_t st, int reps = 3,
+                    int passes = 2) {
+  auto run = [&]() {
+    hipError_t e = launch_family<T>(0, ls[0], g, st);
+    if (e == hipSuccess) e = launch_family<T>(1, ls[1], g, st);
+    return e;
+  };
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return -1.f;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return -1.f;
+  }
+  float best = -1.f;
+  if (run() == hipSuccess) {
+    for (int pass = 0; pass < passes; pass++) {
+      (void)hipEventRecord(e0, st);
+      for (int r = 0; r < reps; r++) (void)run();
+      (void)hipEventRecord(e1, st);
+      if (hipEventSynchronize(e1) != hipSuccess) break;
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      ms /= reps;
+      if (best < 0.f || ms < best) best = ms;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipGetLastError();
+  return best;
+}
+
+// The plan autotuner.  The choices are timed in context: a synthetic batch
+// with the plan's own lengths (every k-th system of each class when the
+// batch holds more than kTuneMaxElems unknowns) is solved with both family
+// launches built from a whole choice vector -- per-class runs in isolation
+// mispredict the mixed launch (their tails and occupancy differ).  Starting
+// from the untuned choices, the classes are visited from the most to the
+// fewest unknowns, and each alternative (other row family; past 64 chunks
+// the other reduced solve) is kept when the whole solve gets faster by more
+// than 2 %.  The timing runs use buffers of their own; the caller's arrays
+// are not touched.
+constexpr long long kTuneMaxElems = 1LL << 27;
+
+template <typename T>
+tridStatus_t tune_plan(const std::vector<int64_t> &off, const std::vector<std::vector<int>> &cls,
+                       bool dbl, RagChoice *choice, std::string &log, hipStream_t st) {
+  long long total = 0;
+  std::vector<long long> unk(kRagClasses, 0);
+  for (int c = 0; c < kRagClasses; c++)
+    for (int id : cls[c]) unk[c] += off[id + 1] - off[id];
+  for (long long u : unk) total += u;
+  if (total == 0) return TRID_STATUS_SUCCESS;
+  const long long stride = std::max(1LL, (total + kTuneMaxElems - 1) / kTuneMaxElems);
+  // the synthetic batch: sampled systems of every class, contiguous
+  std::vector<int64_t> soff{0};
+  std::vector<std::vector<int>> scls(kRagClasses);
+  for (int c = 0; c < kRagClasses; c++)
+    for (size_t i = 0; i < cls[c].size(); i += stride) {
+      const int id = cls[c][i];
+      scls[c].push_back((int)soff.size() - 1);
+      soff.push_back(soff.back() + (off[id + 1] - off[id]));
+    }
+  TuneBufs tb;
+  tridStatus_t s = alloc_synthetic<T>(tb, soff.back(), st);
+  if (!s) s = upload((void **)&tb.off, soff.data(), soff.size() * sizeof(int64_t));
+  unsigned zero[2] = {0, 0};
+  if (!s) s = upload((void **)&tb.counter, zero, sizeof zero);
+  if (s) return s;
+  const RagArgs<T> g{static_cast<const T *>(tb.a), static_cast<const T *>(tb.b),
+                     static_cast<const T *>(tb.c), static_cast<T *>(tb.d), nullptr, 0,
+                     tb.off, nullptr, nullptr, 0, tb.counter};
+  auto eval = [&](const RagChoice *ch, float &ms) -> tridStatus_t {
+    RagLaunch ls[2];
+    tridStatus_t r = TRID_STATUS_SUCCESS;
+    for (int f = 0; f < 2 && !r; f++) {
+      std::vector<int> ids, sys;
+      std::vector<RagItem> items;
+      int red[10];
+      family_lists(scls, ch, f, ids, red);
+      build_items(soff, ids, f, red, sys, items, nullptr);
+      r = make_launch(ls[f], f, dbl, sys, items);
+    }
+    if (!r) {
+      ms = time_launches<T>(ls, g, st);
+      if (ms < 0.f) r = fail(TRID_STATUS_EXECUTION_FAILED, "ragged autotuner: launch failed");
+    }
+    free_launch(ls[0]);
+    free_launch(ls[1]);
+    return r;
+  };
+  float best = 0.f;
+  s = eval(choice, best);
+  if (s) return s;
+  char buf[96];
+  snprintf(buf, sizeof buf, "sample 1/%lld, untuned %.1fus;", stride, 1000.f * best);
+  log += buf;
+  std::vector<int> order;
+  for (int c = 0; c < kRagClasses; c++)
+    if (unk[c] > 0) order.push_back(c);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return unk[x] > unk[y]; });
+  for (int c : order) {
+    std::vector<RagChoice> alts;
+    for (int f = 0; f < 2; f++) {
+      if (!fam_ok(c, f)) continue;
+      for (int red = 0; red < 2; red++) {
+        if (!red_matters(c, f) && red != choice[c].red) continue;
+        if (f == choice[c].fam && red == choice[c].red) continue;
+        alts.push_back({f, red});
+      }
+    }
+    for (const RagChoice &alt : alts) {
+      RagChoice trial[kRagClasses];
+      std::copy(choice, choice + kRagClasses, trial);
+      trial[c] = alt;
+      float ms = 0.f;
+      s = eval(trial, ms);
+      if (s) return s;
+      const bool take = ms < 0.98f * best;
+      snprintf(buf, sizeof buf, " c%d:%d%s=%.1fus%s", c, alt.fam, alt.red ? "w" : "p",
+               1000.f * ms, take ? "*" : "");
+      log += buf;
+      if (take) {
+        best = ms;
+        choice[c] = alt;
+      }
+    }
+  }
+  snprintf(buf, sizeof buf, "; tuned %.1fus", 1000.f * best);
+  log += buf;
+  return TRID_STATUS_SUCCESS;
+}
+
+// Measured cost constants of the solo / one-thread split on the plan's
+// longest system of Lmax rows: (one strided solve, one-thread kernel per row).
+template <typename T>
+tridStatus_t tune_long(long long Lmax, double &solo_us, double &row_us, hipStream_t st) {
+  TuneBufs tb;
+  tridStatus_t s = alloc_synthetic<T>(tb, Lmax, st);
+  int64_t off[2] = {0, Lmax};
+  if (!s) s = upload((void **)&tb.off, off, sizeof off);
+  if (s) return s;
+  int id = 0;
+  long long wso = 0;
+  int *dids = nullptr;
+  long long *dwso = nullptr;
+  T *ws = nullptr;
+  s = upload((void **)&dids, &id, sizeof id);
+  if (!s) s = upload((void **)&dwso, &wso, sizeof wso);
+  if (!s) s = check_hip(hipMalloc((void **)&ws, 2 * (size_t)Lmax * sizeof(T)), "hipMalloc");
+  if (!s) {
+    T *a = static_cast<T *>(tb.a), *b = static_cast<T *>(tb.b), *c = static_cast<T *>(tb.c),
+      *d = static_cast<T *>(tb.d);
+    RagArgs<T> g{a, b, c, d, nullptr, 0, tb.off, nullptr, nullptr, 0, nullptr};
+    int dims[1] = {(int)Lmax}, opts[1] = {0};
+    std::vector<float> tm;
+    auto prep = [&](int) { return true; };
+    auto run = [&](int k) -> hipError_t {
+      if (k == 0) {
+        tridStatus_t r;
+        if constexpr (sizeof(T) == 8)
+          r = tridDmtsvStridedBatch(a, b, c, d, nullptr, 1, 0, dims, dims, opts, 0);
+        else
+          r = tridSmtsvStridedBatch(a, b, c, d, nullptr, 1, 0, dims, dims, opts, 0);
+        return r ? hipErrorUnknown : hipSuccess;
+      }
+      hipLaunchKernelGGL(k_ragged_long<T>, dim3(1), dim3(64), 0, st, g, dids, dwso, 1, ws);
+      return hipGetLastError();
+    };
+    if (pick_fastest(2, prep, run, st, &tm, 3, 1) >= 0) {
+      solo_us = 1000.0 * tm[0] / 3;
+      row_us = 1000.0 * tm[1] / 3 / (double)Lmax;
+    }
+  }
+  (void)hipFree(dids);
+  (void)hipFree(dwso);
+  (void)hipFree(ws);
+  return s;
 }
 
 template <typename T>
@@ -352,28 +767,11 @@ tridStatus_t ragged_solve(tridRaggedPlan_st *p, const T *a, const T *b, const T 
   if (!a || !b || !c || !d || (inc && !u))
     return fail(TRID_STATUS_INVALID_VALUE, "NULL array");
   hipStream_t st = current_stream();
-  RagArgs<T> g{a, b, c, d, u, inc ? 1 : 0, p->offsets, p->sys, p->items, p->nitems, p->counter};
-  if (p->nitems > 0) {
-    // FP64 items have chunks of 5..8 rows, FP32 ones of 9..16
-    if constexpr (sizeof(T) == 8)
-      hipLaunchKernelGGL((k_ragged<T, kRagRmin, kRagRmax>), dim3(p->grid), dim3(kRagThreads),
-                         p->lds, st, g);
-    else
-      hipLaunchKernelGGL((k_ragged<T, kRagR2min, kRagR2max>), dim3(p->grid), dim3(kRagThreads),
-                         p->lds, st, g);
-    tridStatus_t s = check_hip(hipGetLastError(), "ragged kernel launch");
-    if (s) return s;
-  }
-  if (p->nitems2 > 0) {
-    // stream-ordered after the first launch, whose last work-group has reset
-    // the counter
-    RagArgs<T> g2 = g;
-    g2.sys = p->sys2;
-    g2.items = p->items2;
-    g2.nitems = p->nitems2;
-    hipLaunchKernelGGL((k_ragged<T, kRagR2min, kRagR2max>), dim3(p->grid2), dim3(kRagThreads),
-                       p->lds2, st, g2);
-    tridStatus_t s = check_hip(hipGetLastError(), "ragged kernel launch");
+  RagArgs<T> g{a, b, c, d, u, inc ? 1 : 0, p->offsets, nullptr, nullptr, 0, p->counter};
+  // the two launches are stream-ordered: the first one's last work-group
+  // has reset the counter when the second starts
+  for (int f = 0; f < 2; f++) {
+    tridStatus_t s = check_hip(launch_family<T>(f, p->launch[f], g, st), "ragged kernel launch");
     if (s) return s;
   }
   if (p->nlong > 0) {
@@ -402,6 +800,135 @@ tridStatus_t ragged_solve(tridRaggedPlan_st *p, const T *a, const T *b, const T 
   }
   if (sync) return check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
   return TRID_STATUS_SUCCESS;
+}
+
+tridStatus_t create_plan(tridRaggedPlan_st *p, const std::vector<int64_t> &off) {
+  const long long nsys = p->nsys;
+  const bool dbl = p->dbl == 1;
+  hipStream_t st = current_stream();
+  RagChoice forced;
+  {
+    std::lock_guard<std::mutex> lk(g_rag_mu);
+    forced = g_rag_forced;
+  }
+  const bool tune = autotune_mode() == 1 && forced.fam < 0 && forced.red < 0 && !capturing(st);
+  std::string tlog;
+
+  // systems past kRagMaxLen[1] rows: the k longest are solved one by one as
+  // strided batches of one system (a few launches each, bandwidth-bound),
+  // the rest together by the one-thread-per-system kernel (time set by its
+  // longest system); k minimises k * solo + L_(k+1) * row
+  std::vector<std::pair<long long, long long>> over;  // (length, system)
+  std::vector<std::vector<int>> cls(kRagClasses);
+  for (long long k = 0; k < nsys; k++) {
+    const long long L = off[k + 1] - off[k];
+    if (L == 0) continue;
+    if (L > kRagMaxLen[1]) {
+      over.push_back({L, k});
+      continue;
+    }
+    cls[class_of(L)].push_back((int)k);
+  }
+  std::sort(over.begin(), over.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+  for (const auto &o : over)
+    if (o.first > INT32_MAX) return fail(TRID_STATUS_INVALID_VALUE, "system longer than 2^31 - 1 rows");
+  double solo_us = kSoloUs, row_us = kThreadRowUs;
+  if (tune && !over.empty()) {
+    tridStatus_t s = dbl ? tune_long<double>(over[0].first, solo_us, row_us, st)
+                         : tune_long<float>(over[0].first, solo_us, row_us, st);
+    if (s) return s;
+    char buf[96];
+    snprintf(buf, sizeof buf, "long:solo=%.1fus,row=%.3fus ", solo_us, row_us);
+    tlog += buf;
+  }
+  size_t nsolo = 0;
+  {
+    double best = 1e300;
+    for (size_t k = 0; k <= over.size(); k++) {
+      const double t = k * solo_us + (k < over.size() ? over[k].first * row_us : 0.0);
+      if (t < best) {
+        best = t;
+        nsolo = k;
+      }
+    }
+  }
+  std::vector<int> longs;
+  std::vector<long long> long_off;
+  long long lws = 0;
+  for (size_t k = 0; k < over.size(); k++) {
+    if (k < nsolo) {
+      p->solo.push_back({off[over[k].second], (int)over[k].first});
+    } else {
+      longs.push_back((int)over[k].second);
+      long_off.push_back(lws);
+      lws += 2 * over[k].first;
+    }
+  }
+  // keep the one-thread systems in offset order
+  {
+    std::vector<size_t> perm(longs.size());
+    for (size_t i = 0; i < perm.size(); i++) perm[i] = i;
+    std::sort(perm.begin(), perm.end(), [&](size_t x, size_t y) { return longs[x] < longs[y]; });
+    std::vector<int> l2;
+    std::vector<long long> o2;
+    lws = 0;
+    for (size_t i : perm) {
+      l2.push_back(longs[i]);
+      o2.push_back(lws);
+      lws += 2 * (off[longs[i] + 1] - off[longs[i]]);
+    }
+    longs.swap(l2);
+    long_off.swap(o2);
+  }
+
+  // a row family and reduced solve per length class
+  RagChoice choice[kRagClasses];
+  for (int c = 0; c < kRagClasses; c++) {
+    choice[c] = default_choice(c, dbl);
+    if (forced.fam >= 0 && fam_ok(c, forced.fam)) choice[c].fam = forced.fam;
+    if (forced.red >= 0) choice[c].red = forced.red;
+  }
+  if (tune) {
+    tridStatus_t s = dbl ? tune_plan<double>(off, cls, dbl, choice, tlog, st)
+                         : tune_plan<float>(off, cls, dbl, choice, tlog, st);
+    if (s) return s;
+  }
+
+  // items of each family; a family's reduced-solve choice is per chunk count
+  std::string desc;
+  for (int f = 0; f < 2; f++) {
+    std::vector<int> ids;
+    int red[10];
+    family_lists(cls, choice, f, ids, red);
+    std::vector<int> sys;
+    std::vector<RagItem> items;
+    build_items(off, ids, f, red, sys, items, &desc);
+    tridStatus_t s = make_launch(p->launch[f], f, dbl, sys, items);
+    if (s) return s;
+  }
+  if (!longs.empty()) desc += "long(>8192):" + std::to_string(longs.size()) + " ";
+  if (!p->solo.empty()) desc += "solo:" + std::to_string(p->solo.size()) + " ";
+  {
+    std::string cs = "classes:";
+    for (int c = 0; c < kRagClasses; c++)
+      if (!cls[c].empty()) {
+        char buf[48];
+        snprintf(buf, sizeof buf, " c%d=%d%s", c, choice[c].fam, choice[c].red ? "w" : "p");
+        cs += buf;
+      }
+    desc += cs;
+  }
+  if (tune) desc += " tuned: " + tlog;
+  p->text = desc;
+  p->nlong = (int)longs.size();
+  p->long_ws = lws;
+  tridStatus_t s = upload((void **)&p->long_ids, longs.data(), longs.size() * sizeof(int));
+  if (!s) s = upload((void **)&p->long_ws_off, long_off.data(), long_off.size() * sizeof(long long));
+  if (!s) {
+    unsigned zero[2] = {0, 0};
+    s = upload((void **)&p->counter, zero, sizeof zero);
+  }
+  return s;
 }
 
 }  // namespace
@@ -443,165 +970,11 @@ EXTERN_C tridStatus_t tridRaggedPlanCreate(tridRaggedPlan_t *plan,
       return fail(TRID_STATUS_INVALID_VALUE, "offsets must be non-decreasing");
     }
   p->extent = off[nsys];
-  // bins (cps = 2^cl >= L / rmax, R = the shortest of rmin..rmax rows with
-  // cps*R >= L): FP64 chunks of 5..8 rows; FP32 (half the registers per
-  // row) chunks of 9..16 rows, half the chunks and reduced-system levels
-  const int rmin = p->dbl ? kRagRmin : kRagR2min, rmax = p->dbl ? kRagRmax : kRagR2max;
-  const long long max_len = (long long)kRagThreads * rmax;  // first launch
-  auto bin_of = [&](long long L, int &R) {
-    const long long ch = (L + rmax - 1) / rmax;
-    int cl = 0;
-    while ((1LL << cl) < ch) cl++;
-    R = (int)std::max<long long>(rmin, (L + (1LL << cl) - 1) >> cl);
-    return cl;
-  };
-  const int nR = rmax - rmin + 1;
-  std::vector<std::vector<int>> bins(16 * nR);
-  // systems past kRagMaxLen rows: the k longest are solved one by one as
-  // strided batches of one system (a few launches each, bandwidth-bound),
-  // the rest together by the one-thread-per-system kernel (time set by its
-  // longest system); k minimises k * kSoloUs + L_(k+1) * kThreadRowUs
-  std::vector<std::pair<long long, long long>> over;  // (length, system)
-  for (long long k = 0; k < nsys; k++) {
-    const long long L = off[k + 1] - off[k];
-    if (L > kRagLongLen) over.push_back({L, k});
-  }
-  std::sort(over.begin(), over.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
-  size_t nsolo = 0;
-  {
-    double best = 1e300;
-    for (size_t k = 0; k <= over.size(); k++) {
-      const double t = k * kSoloUs + (k < over.size() ? over[k].first * kThreadRowUs : 0.0);
-      if (t < best) {
-        best = t;
-        nsolo = k;
-      }
-    }
-  }
-  std::vector<char> is_solo(nsolo ? nsys : 0, 0);
-  for (size_t k = 0; k < nsolo; k++) {
-    if (over[k].first > INT32_MAX) {
-      free_plan(p);
-      return fail(TRID_STATUS_INVALID_VALUE, "system longer than 2^31 - 1 rows");
-    }
-    is_solo[over[k].second] = 1;
-    p->solo.push_back({off[over[k].second], (int)over[k].first});
-  }
-  std::vector<int> longs;
-  std::vector<long long> long_off;
-  long long lws = 0;
-  constexpr int nR2 = kRagR2max - kRagR2min + 1;
-  std::vector<std::vector<int>> bins2(nR2);  // 512 chunks of R = 9..16 rows
-  for (long long k = 0; k < nsys; k++) {
-    const long long L = off[k + 1] - off[k];
-    if (L == 0 || (nsolo && is_solo[k])) continue;
-    if (L > kRagLongLen) {
-      longs.push_back((int)k);
-      long_off.push_back(lws);
-      lws += 2 * L;
-      continue;
-    }
-    if (L > max_len) {
-      const int R = (int)((L + kRagThreads - 1) / kRagThreads);
-      bins2[R - kRagR2min].push_back((int)k);
-      continue;
-    }
-    int R = 0;
-    const int cl = bin_of(L, R);
-    bins[cl * nR + (R - rmin)].push_back((int)k);
-  }
-  // items, most expensive first (more chunks per system, then longer chunks)
-  std::vector<int> sys;
-  std::vector<RagItem> items;
-  std::string desc;
-  for (int cl = 15; cl >= 0; cl--)
-    for (int R = rmax; R >= rmin; R--) {
-      std::vector<int> &v = bins[cl * nR + (R - rmin)];
-      if (v.empty()) continue;
-      const int per = std::max(1, kRagThreads >> cl);
-      for (size_t i = 0; i < v.size(); i += per) {
-        RagItem it;
-        it.cps_log2 = cl;
-        it.first = (int)sys.size() + (int)i;
-        it.count = (int)std::min<size_t>(per, v.size() - i);
-        it.rows = R;
-        items.push_back(it);
-      }
-      sys.insert(sys.end(), v.begin(), v.end());
-      char buf[96];
-      snprintf(buf, sizeof buf, "R%d/cps%d:%zu ", R, 1 << cl, v.size());
-      desc += buf;
-    }
-  std::vector<int> sys2;
-  std::vector<RagItem> items2;
-  for (int R = kRagR2max; R >= kRagR2min; R--) {
-    const std::vector<int> &v = bins2[R - kRagR2min];
-    if (v.empty()) continue;
-    for (size_t i = 0; i < v.size(); i++) {
-      RagItem it;
-      it.cps_log2 = 9;
-      it.first = (int)(sys2.size() + i);
-      it.count = 1;
-      it.rows = R;
-      items2.push_back(it);
-    }
-    sys2.insert(sys2.end(), v.begin(), v.end());
-    char buf[96];
-    snprintf(buf, sizeof buf, "R%d/cps512:%zu ", R, v.size());
-    desc += buf;
-  }
-  if (!longs.empty()) desc += "long(>8192):" + std::to_string(longs.size()) + " ";
-  if (!p->solo.empty()) desc += "solo:" + std::to_string(p->solo.size());
-  p->text = desc;
-  p->nitems = (int)items.size();
-  p->nlong = (int)longs.size();
-  p->long_ws = lws;
-  const size_t ts = p->dbl ? 8 : 4;
-  // span-group regions (at most 512 groups of G*R + 2E elements, G >= 1);
-  // the LDS reduced solve reuses them (6 values per thread)
-  p->lds = (size_t)kRagThreads * (rmax + 2 * (16 / ts)) * ts;
-  p->lds2 = (size_t)kRagThreads * (kRagR2max + 2 * (16 / ts)) * ts;
-  p->nitems2 = (int)items2.size();
-  auto up = [&](void **dst, const void *src, size_t bytes) -> tridStatus_t {
-    if (bytes == 0) return TRID_STATUS_SUCCESS;
-    tridStatus_t r = check_hip(hipMalloc(dst, bytes), "hipMalloc");
-    if (r) return r;
-    return check_hip(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy");
-  };
-  s = up((void **)&p->sys, sys.data(), sys.size() * sizeof(int));
-  if (!s) s = up((void **)&p->items, items.data(), items.size() * sizeof(RagItem));
-  if (!s) s = up((void **)&p->sys2, sys2.data(), sys2.size() * sizeof(int));
-  if (!s) s = up((void **)&p->items2, items2.data(), items2.size() * sizeof(RagItem));
-  if (!s) s = up((void **)&p->long_ids, longs.data(), longs.size() * sizeof(int));
-  if (!s) s = up((void **)&p->long_ws_off, long_off.data(), long_off.size() * sizeof(long long));
-  if (!s) {
-    unsigned zero[2] = {0, 0};
-    s = up((void **)&p->counter, zero, sizeof zero);
-  }
+  s = create_plan(p, off);
   if (s) {
     free_plan(p);
     return s;
   }
-  // persistent grid: as many work-groups as fit on the device at once
-  int dev = 0, cus = 256;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  int per_cu = 0, per_cu2 = 0;
-  if (p->dbl) {
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, k_ragged<double, kRagRmin, kRagRmax>, kRagThreads, p->lds);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu2, k_ragged<double, kRagR2min, kRagR2max>, kRagThreads, p->lds2);
-  } else {
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, k_ragged<float, kRagR2min, kRagR2max>, kRagThreads, p->lds);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu2, k_ragged<float, kRagR2min, kRagR2max>, kRagThreads, p->lds2);
-  }
-  per_cu = std::max(1, per_cu);
-  per_cu2 = std::max(1, per_cu2);
-  p->grid = std::max(1, std::min(p->nitems, cus * per_cu));
-  p->grid2 = std::max(1, std::min(p->nitems2, cus * per_cu2));
   *plan = p;
   return TRID_STATUS_SUCCESS;
 }
@@ -634,4 +1007,13 @@ EXTERN_C tridStatus_t tridRaggedPlanExtent(tridRaggedPlan_t plan, long long *ele
 
 EXTERN_C const char *tridRaggedPlanDescribe(tridRaggedPlan_t plan) {
   return plan ? plan->text.c_str() : "";
+}
+
+EXTERN_C tridStatus_t tridRaggedForceChoice(int family, int red) {
+  clear_error();
+  if (family < -1 || family > 1 || red < -1 || red > 1)
+    return fail(TRID_STATUS_INVALID_VALUE, "family and red must be -1, 0 or 1");
+  std::lock_guard<std::mutex> lk(g_rag_mu);
+  g_rag_forced = RagChoice{family, red};
+  return TRID_STATUS_SUCCESS;
 }

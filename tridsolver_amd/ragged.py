@@ -29,6 +29,13 @@ def check_arrays(extent: int, dtype, a, b, c, d, u=None, inc: bool = False) -> N
             raise ValueError(f"{name} holds {t.numel()} elements; the offsets name {extent}")
 
 
+def force_choice(family: int = -1, red: int = -1) -> None:
+    """Plans created afterwards use row family `family` (0: chunks of 5..8
+    rows, 1: 9..16) and reduced solve `red` past 64 chunks (0: PCR through
+    LDS, 1: per-wave PCR and a sweep across waves); -1 = automatic."""
+    _check(lib().tridRaggedForceChoice(int(family), int(red)), "tridRaggedForceChoice")
+
+
 class RaggedPlan:
     def __init__(self, offsets, dtype):
         import torch
