@@ -62,4 +62,23 @@ EXTERN_C tridStatus_t tridForceWideIndex(int on);
 EXTERN_C tridStatus_t tridDebugReciprocal(const void *x, void *y, long long n,
                                           int is_double);
 
+/* No-allocation form of tridDmtsvStridedBatch (same arguments, same
+ * results).  It has no counterpart in the reference; it is for callers that
+ * cannot allocate device memory: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  It calls no hipMalloc, hipFree or
+ * allocating library routine and never autotunes: the untuned schedule of
+ * tridDmtsvStridedBatch (autotuning off) runs, bit for bit the same
+ * kernels.  The scratch of split-line solves (systems longer than 8192 rows;
+ * solvedim >= 1 from 2048 rows) and of the forced one-thread-per-system
+ * kernel comes from a static device array of TRID_NOALLOC_SCRATCH_BYTES that
+ * each device holds until the process ends; a solve that needs more returns
+ * TRID_STATUS_INVALID_VALUE (the last error names both sizes) before
+ * launching anything.  Every call on one device shares that array: calls that
+ * overlap on one device (other streams or threads) race. */
+#define TRID_NOALLOC_SCRATCH_BYTES (64u << 20)
+EXTERN_C tridStatus_t tridDmtsvStridedBatchNoAlloc(const double *a, const double *b,
+                                                   const double *c, double *d, double *u,
+                                                   int ndim, int solvedim, int *dims,
+                                                   int *pads, int *opts, int sync);
+
 #endif

@@ -31,6 +31,7 @@ SIGNATURES = {
     "tridForceConfig": (ST, [I, I, I, I]),
     "tridForceWideIndex": (ST, [I]),
     "tridDebugReciprocal": (ST, [P, P, C.c_longlong, I]),
+    "tridDmtsvStridedBatchNoAlloc": (ST, [P, P, P, P, P, I, I, IP, IP, IP, I]),
     # trid_adi.h
     "tridAdiPlanCreate": (ST, [C.POINTER(P), I, I, I, I, C.c_double, I]),
     "tridAdiPlanDestroy": (ST, [P]),

@@ -379,7 +379,8 @@ template <typename T, int INC>
 tridStatus_t solve(const T *a, const int *a_pads, const T *b,
                    const int *b_pads, const T *c, const int *c_pads, T *d,
                    const int *d_pads, T *u, const int *u_pads, int ndim,
-                   int solvedim, const int *dims, const int *opts, int sync) {
+                   int solvedim, const int *dims, const int *opts, int sync,
+                   void *fixed = nullptr, size_t fixed_bytes = 0) {
   (void)opts;  // accepted for compatibility; the kernel choice is automatic
   clear_error();
   if (!dims) return fail(TRID_STATUS_INVALID_VALUE, "dims is NULL");
@@ -496,8 +497,9 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
     // `st` is being captured takes the untuned default (tuning synchronises
     // and times launches, which a capture cannot hold) and is tuned at its
     // first call outside a capture
+    // (a call with fixed scratch never tunes: tuning allocates)
     bool have = false;
-    const bool tune = autotune_mode() == 1;
+    const bool tune = autotune_mode() == 1 && !fixed;
     auto key = std::make_tuple((int)sizeof(T), INC, order, D.N, D.nsys,
                                vec * 4 + (int)gvec * 2 + (int)dense, D.sstr[3],
                                D.nother > 0 ? D.ostr[3][0] : 0LL);
@@ -530,7 +532,7 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
     return fail(TRID_STATUS_INVALID_VALUE, "split-line solve: too many systems");
   cfg.idx32 = idx32;
   set_last_config(cfg_name(cfg, sizeof(T)));
-  tridStatus_t s = run_cfg<T, INC>(cfg, a, b, c, d, u, D, st);
+  tridStatus_t s = run_cfg<T, INC>(cfg, a, b, c, d, u, D, st, fixed, fixed_bytes);
   if (s != TRID_STATUS_SUCCESS) return s;
   if (sync) return check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
   return TRID_STATUS_SUCCESS;
@@ -643,4 +645,54 @@ EXTERN_C tridStatus_t tridSmtsvStridedBatchPaddedInc(
     int sync) {
   return solve<float, 1>(a, a_pads, b, b_pads, c, c_pads, d, d_pads, u,
                          u_pads, ndim, solvedim, dims, opts, sync);
+}
+
+// ---------------------------------------------------------------------------
+// no-allocation entry point
+//
+// The scratch of split-line and one-thread-per-system solves comes from this
+// file-scope device array instead of the per-stream workspace.  Every device
+// holds its own copy at its own address; host code never passes the array's
+// name to a kernel (on the host it is only a stand-in for the symbol) but the
+// address hipGetSymbolAddress returns for the current device, fetched once
+// per device.  Nothing in it needs clearing between calls: every kernel that
+// uses it writes each value before reading it.  Calls that overlap on one
+// device (two streams, two threads) share the array and race.
+static __device__ __align__(256) unsigned char scratch[TRID_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_ptr;  // device -> address of `scratch` there
+
+tridStatus_t scratch_address(void **out) {
+  int dev = 0;
+  tridStatus_t s = tridhip::check_hip(hipGetDevice(&dev), "hipGetDevice");
+  if (s) return s;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) {
+    *out = it->second;
+    return TRID_STATUS_SUCCESS;
+  }
+  void *ptr = nullptr;
+  s = tridhip::check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)),
+                         "hipGetSymbolAddress(scratch)");
+  if (s) return s;
+  g_scratch_ptr[dev] = ptr;
+  *out = ptr;
+  return TRID_STATUS_SUCCESS;
+}
+}  // namespace
+
+EXTERN_C tridStatus_t tridDmtsvStridedBatchNoAlloc(const double *a, const double *b,
+                                                   const double *c, double *d, double *u,
+                                                   int ndim, int solvedim, int *dims,
+                                                   int *pads, int *opts, int sync) {
+  (void)u;
+  tridhip::clear_error();
+  void *ptr = nullptr;
+  tridStatus_t s = scratch_address(&ptr);
+  if (s) return s;
+  return solve<double, 0>(a, pads, b, pads, c, pads, d, pads, nullptr, pads, ndim, solvedim,
+                          dims, opts, sync, ptr, TRID_NOALLOC_SCRATCH_BYTES);
 }
