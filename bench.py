@@ -73,6 +73,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
+    ap.add_argument("--no-baselines", action="store_true",
+                    help="skip the ragged workload's per-length launches and graph (for "
+                         "profiler runs that should hold the ragged kernels only)")
     ap.add_argument("--extras", default="plain,ragged,vadi,block,split,adi_sizes",
                     help="comma-separated subset of plain,ragged,vadi,block,split,adi_sizes")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -80,22 +83,41 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_from_profiles(kernel_key: str, config: str):
+def kernel_of_config(cfg: str, dtype: str, inc: int, coef: str):
+    """The kernel (template instance) that runs a configuration string of the
+    library (tridGetLastConfig, an ADI plan's stage), as rocprofv3 names it."""
+    import re
+    T = "double" if dtype == "f64" else "float"
+    m = re.search(r"cc<R(\d+),cps\d+((?:,\w+)*)>", cfg)
+    if m:
+        ns = 2 if "pair" in m.group(2).split(",") else 1
+        return f"k_cc_sweep<{T}, {inc}, {m.group(1)}, unsigned int, {ns}>"
+    m = re.search(r"part<\w+,o(\d),R(\d+),cps(\d+),spb\d+((?:,\w+)*)>", cfg)
+    if m:
+        fl = m.group(4).split(",")
+        staged = "staged" in fl
+        vec = 4 if "gstaged" in fl else (3 if staged and int(m.group(3)) > 64 else
+                                         (2 if staged else (1 if "vec" in fl else 0)))
+        return f"k_part<{T}, {inc}, {m.group(1)}, {m.group(2)}, {vec}, unsigned int, {coef}<{T}> >"
+    m = re.search(r"rhs\+x\(fused,R(\d+)\)", cfg)
+    if m:
+        return f"k_adi_rhs_x<{T}, {m.group(1)}>"
+    return None
+
+
+def traffic_from_profiles(key: str):
     """HBM bytes per launch of a kernel as measured by rocprofv3 --pmc FETCH_SIZE and
     WRITE_SIZE (calibrated against a plain device copy), from profiles/traffic.json;
-    keys are "<workload>|<kernel>|<dtype>".  Returns (bytes or None, note): the
-    entry counts only when it was measured on the configuration this run used."""
+    keys are "<workload><n>|<dtype>|<kernel>" (profiles/traffic_counters.md).
+    Returns (bytes or None, note)."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p) as f:
             t = json.load(f)
-        v = t.get(kernel_key)
+        v = t.get(key)
         if v is None:
-            return None, "no counter measurement for this kernel"
-        if v.get("config") != config:
-            return None, (f"counters were measured on {v.get('config')!r}; this run used "
-                          f"{config!r}")
-        return float(v["bytes_per_launch"]), f"rocprofv3 counters of {v.get('kernel')}"
+            return None, f"no counter measurement of {key.split('|')[-1]} on this workload"
+        return float(v["bytes_per_launch"]), f"rocprofv3 counters of {v['kernel']} ({v.get('note', '')})"
     except (OSError, ValueError, KeyError):
         return None, "profiles/traffic.json unreadable"
 
@@ -479,6 +501,9 @@ def extra_split(torch, np, ta, cpu, reps=10):
     return out
 
 
+RAGGED_BASELINES = True  # --no-baselines sets False
+
+
 def extra_ragged(torch, np, ta, cpu, reps=10):
     import ctypes as C
     from tridsolver_amd.ragged import RaggedPlan
@@ -524,40 +549,43 @@ def extra_ragged(torch, np, ta, cpu, reps=10):
                         None, 0)
                 if st:
                     raise RuntimeError(ta.last_error())
-        per_length()
-        torch.cuda.synchronize()
-        diff = float((dev[3][t_idx] - srt[3]).abs().max() / srt[3].abs().max())
+        diff = t_eager = t_graph = None
+        if RAGGED_BASELINES:
+            per_length()
+            torch.cuda.synchronize()
+            diff = float((dev[3][t_idx] - srt[3]).abs().max() / srt[3].abs().max())
         dev[3].copy_(d0)
         srt[3].copy_(sd0)
         t_rag = _timed(torch, lambda: plan.solve(dev[0], dev[1], dev[2], dev[3], sync=0), reps)
-        t_eager = _timed(torch, per_length, 3)
-        # the per-length launches captured once into a graph and replayed
-        g = torch.cuda.CUDAGraph()
-        s = torch.cuda.Stream()
-        t_graph = None
-        try:
-            with torch.cuda.graph(g, stream=s):
-                ta.set_stream(s)
-                per_length()
-            ta.set_stream(None)
-            g.replay()
-            torch.cuda.synchronize()
-            t_graph = _timed(torch, g.replay, reps)
-        finally:
-            ta.set_stream(None)
+        if RAGGED_BASELINES:
+            t_eager = _timed(torch, per_length, 3)
+            # the per-length launches captured once into a graph and replayed
+            g = torch.cuda.CUDAGraph()
+            s = torch.cuda.Stream()
+            try:
+                with torch.cuda.graph(g, stream=s):
+                    ta.set_stream(s)
+                    per_length()
+                ta.set_stream(None)
+                g.replay()
+                torch.cuda.synchronize()
+                t_graph = _timed(torch, g.replay, reps)
+            finally:
+                ta.set_stream(None)
+            del g
         gbs = (5 * ws * n + 8 * (nsys + 1)) / (t_rag * 1e-3) / 1e9
         out[name] = {
             "ragged_ms": round(t_rag, 4), "Gunknowns_s": round(n / (t_rag * 1e-3) / 1e9, 2),
             "GB_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
-            "per_length_eager_ms": round(t_eager, 3),
+            "per_length_eager_ms": None if t_eager is None else round(t_eager, 3),
             "per_length_graph_ms": None if t_graph is None else round(t_graph, 3),
-            "speedup_vs_eager": round(t_eager / t_rag, 2),
+            "speedup_vs_eager": None if t_eager is None else round(t_eager / t_rag, 2),
             "speedup_vs_graph": None if t_graph is None else round(t_graph / t_rag, 2),
             "max_rel_diff_ragged_vs_per_length": diff,
             "plan_create_s": round(t_plan, 3),
             "plan": plan.describe()}
         plan.close()
-        del dev, srt, g
+        del dev, srt
         torch.cuda.empty_cache()
     if cpu:
         import oracle
@@ -812,6 +840,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": 1 << 30, "launches": 5}))
         return
+    global RAGGED_BASELINES
+    RAGGED_BASELINES = not args.no_baselines
     ta.set_stream(None)  # null stream == torch's default stream
     ta.set_autotune(args.autotune)
     W = (adi_workload if args.workload == "adi" else plain_workload)(args, torch, np, api, ta)
@@ -885,8 +915,11 @@ def main():
     out["roofline"]["config"] = dom_cfg
     if "plan" in W:
         out["config"]["plan"] = W["plan"]
+    kname = kernel_of_config(dom_cfg, args.dtype, 1 if W["kernels"][dom] == "sweep_z_inc" else 0,
+                             "CoefConst" if args.workload == "adi" else "CoefArrays")
+    out["roofline"]["kernel_name"] = kname
     out["roofline"]["traffic"], out["roofline"]["traffic_source"] = traffic_from_profiles(
-        f"{args.workload}|{W['kernels'][dom]}|{args.dtype}", dom_cfg)
+        f"{args.workload}{args.n}|{args.dtype}|{kname}")
     if not args.no_cpu:
         # the GPU result of the same work the oracle runs, then the oracle
         # (outside every timed region)

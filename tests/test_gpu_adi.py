@@ -154,6 +154,38 @@ def test_adi_every_sweep_layout(shape, dt):
     print(f"{shape} {dt.__name__}: {ran} layouts")
 
 
+@pytest.mark.parametrize("shape", [(64, 40, 24), (256, 24, 30), (32, 8, 5)])
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_adi_every_fused_x_layout(shape, dt):
+    """The fused right-hand side + x stage pinned (tridAdiPlanSetSweep dir 0)
+    to every chunk length and both block orders (the XCD-contiguous order
+    included), against the oracle; rows that do not fit are rejected."""
+    import torch
+    nx, ny, nz = shape
+    u0 = rand_init(nx, ny, nz, dt, seed=nx + ny)
+    exp = u0.ravel().copy()
+    assert oracle.adi(exp, nx, ny, nz, 3, 1.0, 1) == 0
+    ran = 0
+    for rows in (4, 8, 16):
+        for xcd in (0, 1):
+            plan = AdiPlan(nx, ny, nz, torch_dt(dt), 1.0, 1)
+            assert "fused" in plan.describe()
+            cps = nx // rows
+            fits = nx % rows == 0 and cps <= 64 and ny % (64 // cps) == 0
+            if not fits:
+                with pytest.raises(ta.TridError):
+                    plan.set_sweep(0, rows, 0, xcd)
+                continue
+            plan.set_sweep(0, rows, 0, xcd)
+            assert f"fused,R{rows}{',xcd' if xcd else ''})" in plan.describe()
+            u = torch.from_numpy(u0.copy()).to("cuda")
+            plan.step(u, torch.empty_like(u), 3)
+            err = oracle.rel_err(exp, u.cpu().numpy().ravel())
+            assert err <= tol(dt), f"{shape} R{rows} xcd {xcd} [{plan.describe()}]: {err:.2e}"
+            ran += 1
+    assert ran >= 1
+
+
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 def test_adi_step_graph_matches_single_steps(dt):
     """tridAdiStep with several steps launches a cached graph of up to 64

@@ -92,19 +92,27 @@ __device__ __forceinline__ T rhs_point(T l, T r, T ym, T yp, T zm, T zp, T c,
 // (y-1, y+1, z-1, z+1: contiguous spans at -nx, +nx, -nx*ny, +nx*ny) are
 // loaded as whole spans; the x-neighbours come from the staging buffer.
 // HBM traffic: u once (neighbour spans hit L2 / MALL) and du once.
+// xcd = 1: work-group b takes line block (b mod 8) * (B / 8) + b / 8 (B the
+// grid rounded down to a multiple of 8), so that each XCD -- work-groups are
+// dealt to the 8 XCDs round-robin -- sweeps one contiguous range of planes
+// and finds the neighbour planes of its lines in its own L2.
 template <typename T, int R>
 __global__ void __launch_bounds__(256)
     k_adi_rhs_x(const T *__restrict__ u, T *__restrict__ du, int nx, int ny,
-                int nz, T lam, CoefConst<T> cf, int cps_log2) {
+                int nz, T lam, CoefConst<T> cf, int cps_log2, int xcd) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int tid = threadIdx.x, lane = tid & (kWave - 1);
   const int cps = 1 << cps_log2;
   const int k = tid & (cps - 1);
   const int lines_per_block = 256 >> cps_log2;
   const long long nlines = (long long)ny * nz;
-  const long long line = (long long)blockIdx.x * lines_per_block + (tid >> cps_log2);
-  const long long wline0 =
-      (long long)blockIdx.x * lines_per_block + ((tid & ~(kWave - 1)) >> cps_log2);
+  long long blk = blockIdx.x;
+  if (xcd) {
+    const unsigned nb8 = gridDim.x & ~7u;
+    if (blockIdx.x < nb8) blk = (long long)(blockIdx.x & 7u) * (nb8 >> 3) + (blockIdx.x >> 3);
+  }
+  const long long line = blk * lines_per_block + (tid >> cps_log2);
+  const long long wline0 = blk * lines_per_block + ((tid & ~(kWave - 1)) >> cps_log2);
   const bool wave_active = wline0 < nlines;
   const long long nxy = (long long)nx * ny;
   const int kw = (int)(wline0 / ny);
@@ -210,6 +218,7 @@ struct tridAdiPlan_st {
   PartCfg cfg_x_unaligned;
   int fused_R = 0;          // >0: fused RHS + x sweep with this chunk length
   int fused_cps_log2 = 0;
+  int fused_xcd = 0;        // XCD-contiguous block order of the fused kernel
   // y and z sweeps with precomputed factors (trid_cc.hpp): chunk length (0 =
   // the generic kernel), chunks per system and the factor table
   int cc_R[3] = {0, 0, 0};
@@ -247,13 +256,16 @@ hipError_t launch_fused(tridAdiPlan_st *p, T *u, T *du, hipStream_t st) {
   const size_t lds = 256 * (size_t)p->fused_R * sizeof(T);
   if (p->fused_R == 8)
     hipLaunchKernelGGL((k_adi_rhs_x<T, 8>), dim3(blocks), dim3(256), lds, st, u, du,
-                       p->nx, p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2);
+                       p->nx, p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2,
+                       p->fused_xcd);
   else if (p->fused_R == 4)
     hipLaunchKernelGGL((k_adi_rhs_x<T, 4>), dim3(blocks), dim3(256), lds, st, u, du,
-                       p->nx, p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2);
+                       p->nx, p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2,
+                       p->fused_xcd);
   else
     hipLaunchKernelGGL((k_adi_rhs_x<T, 16>), dim3(blocks), dim3(256), lds, st, u, du,
-                       p->nx, p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2);
+                       p->nx, p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2,
+                       p->fused_xcd);
   return hipGetLastError();
 }
 
@@ -606,13 +618,16 @@ void tune_plan_t(tridAdiPlan_st *p, T *u, T *du) {
     for (void *F : tables)
       if (F != p->cc_F[dir]) (void)hipFree(F);
   }
-  // chunk length of the fused RHS + x kernel
+  // chunk length and block order of the fused RHS + x kernel (the untuned
+  // choice first)
   if (p->fused_R) {
-    std::vector<int> rows;
+    std::vector<std::pair<int, int>> ch{{p->fused_R, p->fused_xcd}};
     for (int R : {4, 8, 16})
-      if (fused_rows_ok(p, R)) rows.push_back(R);
+      for (int x : {0, 1})
+        if (fused_rows_ok(p, R) && !(R == p->fused_R && x == p->fused_xcd)) ch.push_back({R, x});
     auto cand = [&](int i) {
-      set_fused(p, rows[i]);
+      set_fused(p, ch[i].first);
+      p->fused_xcd = ch[i].second;
       return launch_fused<T>(p, u, du, st);
     };
     auto suffix = [&]() {
@@ -620,11 +635,13 @@ void tune_plan_t(tridAdiPlan_st *p, T *u, T *du) {
       stage(3);
     };
     std::vector<float> tm;
-    const int i = pick_in_step((int)rows.size(), []() {}, cand, suffix, st, &tm);
+    const int i = pick_in_step((int)ch.size(), []() {}, cand, suffix, st, &tm);
     if (tune_log())
-      for (size_t c = 0; c < rows.size(); c++)
-        fprintf(stderr, "[tridhip tune] rhs+x fused R%d: %.4f ms\n", rows[c], tm[c]);
-    set_fused(p, i >= 0 ? rows[i] : rows[0]);
+      for (size_t c = 0; c < ch.size(); c++)
+        fprintf(stderr, "[tridhip tune] rhs+x fused R%d%s: %.4f ms\n", ch[c].first,
+                ch[c].second ? ",xcd" : "", tm[c]);
+    set_fused(p, ch[i >= 0 ? i : 0].first);
+    p->fused_xcd = ch[i >= 0 ? i : 0].second;
   }
   (void)hipMemcpyAsync(u, save, bytes, hipMemcpyDeviceToDevice, st);
   (void)hipMemcpyAsync(du, save + npts, bytes, hipMemcpyDeviceToDevice, st);
@@ -642,7 +659,8 @@ std::string plan_text(const tridAdiPlan_st *p) {
            (p->cc_pair[dir] ? ",pair>" : ">");
   };
   std::string fx = "rhs<";
-  if (p->fused_R) fx = "rhs+x(fused,R" + std::to_string(p->fused_R) + ")<";
+  if (p->fused_R)
+    fx = "rhs+x(fused,R" + std::to_string(p->fused_R) + (p->fused_xcd ? ",xcd" : "") + ")<";
   return fx +
          (p->dbl ? "double" : "float") + "> x:" + cfg_name(p->cfg[0], ts) +
          " y:" + sweep_name(1) + " z:" + sweep_name(2);
@@ -902,9 +920,22 @@ EXTERN_C const char *tridAdiPlanDescribe(tridAdiPlan_t p) {
 EXTERN_C tridStatus_t tridAdiPlanSetSweep(tridAdiPlan_t p, int dir, int rows, int pair,
                                           int xcd) {
   if (!p) return fail(TRID_STATUS_INVALID_VALUE, "NULL plan");
-  if (dir != 1 && dir != 2) return fail(TRID_STATUS_INVALID_VALUE, "dir must be 1 (y) or 2 (z)");
+  if (dir < 0 || dir > 2) return fail(TRID_STATUS_INVALID_VALUE, "dir must be 0, 1 or 2");
   if ((pair != 0 && pair != 1) || (xcd != 0 && xcd != 1))
     return fail(TRID_STATUS_INVALID_VALUE, "pair and xcd must be 0 or 1");
+  if (dir == 0) {
+    // the fused right-hand side + x sweep: chunk length and block order
+    if (!p->fused_R) return fail(TRID_STATUS_INVALID_VALUE, "the plan has no fused x stage");
+    if (pair || !fused_rows_ok(p, rows))
+      return fail(TRID_STATUS_INVALID_VALUE, "rows do not fit the fused x stage (pair = 0)");
+    drop_graphs(p, true);
+    set_fused(p, rows);
+    p->fused_xcd = xcd;
+    p->tune_pending = false;
+    p->text = plan_text(p);
+    clear_error();
+    return TRID_STATUS_SUCCESS;
+  }
   void *F = nullptr;
   int cl = 0;
   if (rows != 0) {
