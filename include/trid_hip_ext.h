@@ -26,6 +26,16 @@ EXTERN_C void *tridGetStream(void);
  * variable TRIDHIP_AUTOTUNE=1 sets 1 at load time. */
 EXTERN_C tridStatus_t tridSetAutotune(int mode);
 
+/* Diagnostic: the calling thread's next kernel launch of the library records
+ * start / stop (hipEvent_t, either may be NULL) around its own execution
+ * (hipExtLaunchKernel timestamps the dispatch packet), so that
+ * hipEventElapsedTime(start, stop) is the kernel's duration without the
+ * dispatch latency an event recorded before the call adds.  One launch
+ * consumes them.  The kernels of strided solves (not split-line) and of the
+ * ADI stages honour them; the fused ADI x stage, its y and z sweeps each
+ * launch one kernel per tridAdiStage call. */
+EXTERN_C tridStatus_t tridSetLaunchEvents(void *start, void *stop);
+
 /* Short text of the last failing call on this thread ("" if none). */
 EXTERN_C const char *tridGetLastError(void);
 EXTERN_C const char *tridGetStatusString(tridStatus_t status);

@@ -23,6 +23,7 @@ SIGNATURES = {
     "tridSmtsvStridedBatchPaddedInc": (ST, [P, IP, P, IP, P, IP, P, IP, P, IP, I, I, IP, IP, I]),
     # trid_hip_ext.h
     "tridSetStream": (ST, [P]),
+    "tridSetLaunchEvents": (ST, [P, P]),
     "tridGetStream": (P, []),
     "tridSetAutotune": (ST, [I]),
     "tridGetLastError": (C.c_char_p, []),

@@ -255,15 +255,15 @@ hipError_t launch_fused(tridAdiPlan_st *p, T *u, T *du, hipStream_t st) {
   const unsigned blocks = (unsigned)((nlines + lines_per_block - 1) / lines_per_block);
   const size_t lds = 256 * (size_t)p->fused_R * sizeof(T);
   if (p->fused_R == 8)
-    hipLaunchKernelGGL((k_adi_rhs_x<T, 8>), dim3(blocks), dim3(256), lds, st, u, du,
+    (void)launch_kernel(k_adi_rhs_x<T, 8>, dim3(blocks), dim3(256), (std::uint32_t)lds, st, u, du,
                        p->nx, p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2,
                        p->fused_xcd);
   else if (p->fused_R == 4)
-    hipLaunchKernelGGL((k_adi_rhs_x<T, 4>), dim3(blocks), dim3(256), lds, st, u, du,
+    (void)launch_kernel(k_adi_rhs_x<T, 4>, dim3(blocks), dim3(256), (std::uint32_t)lds, st, u, du,
                        p->nx, p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2,
                        p->fused_xcd);
   else
-    hipLaunchKernelGGL((k_adi_rhs_x<T, 16>), dim3(blocks), dim3(256), lds, st, u, du,
+    (void)launch_kernel(k_adi_rhs_x<T, 16>, dim3(blocks), dim3(256), (std::uint32_t)lds, st, u, du,
                        p->nx, p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2,
                        p->fused_xcd);
   return hipGetLastError();
@@ -341,9 +341,9 @@ hipError_t launch_cc_t(int R, int cps_log2, int xcd, int pair, const void *F, co
   const size_t lds = 2 * (size_t)ns * kWave * cps * sizeof(T);
   const dim3 grid(blocks), block(kWave * cps);
   const T *f = static_cast<const T *>(F);
-#define TRID_CC(RR, II, NN)                                                                \
-  hipLaunchKernelGGL((k_cc_sweep<T, INC, RR, II, NN>), grid, block, lds, st, f, d, u, D,   \
-                     cps_log2, xcd)
+#define TRID_CC(RR, II, NN)                                                               \
+  (void)launch_kernel(k_cc_sweep<T, INC, RR, II, NN>, grid, block, (std::uint32_t)lds, st, f, \
+                      d, u, D, cps_log2, xcd)
 #define TRID_CC_R(II, NN)      \
   if (R == 8) TRID_CC(8, II, NN);       \
   else if (R == 16) TRID_CC(16, II, NN); \

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "trid_part.hpp"
+#include "trid_runtime.hpp"
 
 namespace tridhip {
 
@@ -84,14 +85,12 @@ hipError_t launch_part_t(const PartCfg &cfg, const Coef &coef, T *d, T *u,
     lds = (size_t)(threads / kWave) * (kWave * R + 16 / sizeof(T) + (cps > kWave ? 12 : 0)) *
           sizeof(T);
   if (cfg.idx32)
-    hipLaunchKernelGGL((k_part<T, INC, ORDER, R, VEC, uint32_t, Coef>),
-                       dim3((unsigned)blocks), dim3(threads), lds, st, coef, d,
-                       u, D, cfg.cps_log2, cfg.spb_log2, cfg.xcd, out);
-  else
-    hipLaunchKernelGGL((k_part<T, INC, ORDER, R, VEC, long long, Coef>),
-                       dim3((unsigned)blocks), dim3(threads), lds, st, coef, d,
-                       u, D, cfg.cps_log2, cfg.spb_log2, cfg.xcd, out);
-  return hipGetLastError();
+    return launch_kernel(k_part<T, INC, ORDER, R, VEC, uint32_t, Coef>, dim3((unsigned)blocks),
+                         dim3(threads), (std::uint32_t)lds, st, coef, d, u, D, cfg.cps_log2,
+                         cfg.spb_log2, cfg.xcd, out);
+  return launch_kernel(k_part<T, INC, ORDER, R, VEC, long long, Coef>, dim3((unsigned)blocks),
+                       dim3(threads), (std::uint32_t)lds, st, coef, d, u, D, cfg.cps_log2,
+                       cfg.spb_log2, cfg.xcd, out);
 }
 
 // Launches the partitioned kernel for any configuration of the table.

@@ -23,6 +23,7 @@ std::mutex g_mu;
 ForcedConfig g_forced;
 thread_local std::string t_error;
 thread_local std::string t_config;
+thread_local LaunchEvents t_events;
 
 struct DevWs {
   void *ptr = nullptr;
@@ -37,6 +38,12 @@ std::vector<void *> g_retired;
 }  // namespace
 
 hipStream_t current_stream() { return g_stream.load(); }
+
+LaunchEvents take_launch_events() {
+  const LaunchEvents e = t_events;
+  t_events = LaunchEvents{};
+  return e;
+}
 
 tridStatus_t fail(tridStatus_t st, const std::string &msg) {
   t_error = msg;
@@ -126,6 +133,12 @@ ForcedConfig forced_config() {
 }  // namespace tridhip
 
 using namespace tridhip;
+
+EXTERN_C tridStatus_t tridSetLaunchEvents(void *start, void *stop) {
+  tridhip::t_events.start = static_cast<hipEvent_t>(start);
+  tridhip::t_events.stop = static_cast<hipEvent_t>(stop);
+  return TRID_STATUS_SUCCESS;
+}
 
 EXTERN_C tridStatus_t tridSetStream(void *stream) {
   g_stream.store(reinterpret_cast<hipStream_t>(stream));

@@ -2,9 +2,11 @@
 // points: current stream, per-thread error text, per-device workspace cache.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <cstdint>
 #include <string>
 
 #include "../../include/trid_common.h"
@@ -37,6 +39,28 @@ bool capturing(hipStream_t st);
 void *workspace(size_t bytes, hipStream_t st);
 
 int autotune_mode();
+
+// Events that the calling thread's next kernel launch of the library records
+// (tridSetLaunchEvents): hipExtLaunchKernel timestamps the dispatch itself,
+// so start/stop bracket exactly the kernel's execution.  Taking them clears
+// them.
+struct LaunchEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+LaunchEvents take_launch_events();
+
+// Launches `kernel` on `st`; when launch events are pending (see above) the
+// launch records them.  Returns hipGetLastError().
+template <typename... Args, typename F = void (*)(Args...)>
+inline hipError_t launch_kernel(F kernel, dim3 grid, dim3 block, std::uint32_t lds,
+                                hipStream_t st, Args... args) {
+  const LaunchEvents ev = take_launch_events();
+  if (ev.start || ev.stop)
+    hipExtLaunchKernelGGL(kernel, grid, block, lds, st, ev.start, ev.stop, 0, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, lds, st, args...);
+  return hipGetLastError();
+}
 
 // True while tridForceWideIndex(1) is in effect: the kernels then use 64-bit
 // element offsets and 64-bit system-index arithmetic whatever the sizes.

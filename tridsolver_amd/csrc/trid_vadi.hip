@@ -257,37 +257,56 @@ __global__ void __launch_bounds__(256)
   const int size[3] = {g.nx, g.ny, g.nz};
   const long long st[3] = {1, g.nx, g.nxy};
   const bool inp = grid_interior(g, c[0], c[1], c[2]);
-  const double up = U[p], kp = K[p];
-  const double rp = inp ? LX[p] : 0.0;
+  // every load is issued before the first use: neighbours off the grid read
+  // the point itself and their terms are masked (a straight-line gather of
+  // 38 independent loads instead of a chain of dependent branches)
+  const double up = U[p], kp = K[p], lxp = LX[p];
+  const double xc[3] = {P[p], Q[p], U1[p]};
+  const double lc[3] = {lxp, LY[p], LZ[p]};
+  double un[3][2], kn[3][2], rn[3][2], xn[3][2], ln[3][2];
+  bool has[3][2], inn[3][2];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+#pragma unroll
+    for (int sd = 0; sd < 2; sd++) {
+      const int t = c[a] + (sd ? 1 : -1);
+      has[a][sd] = sd ? t < size[a] : t >= 0;
+      const long long q = has[a][sd] ? p + (sd ? st[a] : -st[a]) : p;
+      int cc[3] = {c[0], c[1], c[2]};
+      cc[a] = t;
+      inn[a][sd] = has[a][sd] && grid_interior(g, cc[0], cc[1], cc[2]);
+      un[a][sd] = U[q];
+      kn[a][sd] = K[q];
+      rn[a][sd] = LX[q];
+      xn[a][sd] = a == 0 ? P[q] : (a == 1 ? Q[q] : U1[q]);
+      ln[a][sd] = a == 0 ? rn[a][sd] : (a == 1 ? LY[q] : LZ[q]);
+    }
+  }
+  const double ub0 = UB[p], gk0 = GK[p];
+  const double rp = inp ? lxp : 0.0;
   double du = 0.0, dk = 0.0;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    const double *__restrict__ L = a == 0 ? LX : (a == 1 ? LY : LZ);
-    const double xp = a == 0 ? P[p] : (a == 1 ? Q[p] : U1[p] - up);
-    const double lp = inp ? L[p] : 0.0;
-    const int t = c[a];
+    const double xp = a == 2 ? xc[2] - up : xc[a];
+    const double lp = inp ? lc[a] : 0.0;
 #pragma unroll
-    for (int side = -1; side <= 1; side += 2) {
-      if (side < 0 ? t < 1 : t + 1 >= size[a]) continue;
-      const long long pn = p + side * st[a];
-      c[a] = t + side;
-      const bool inn = grid_interior(g, c[0], c[1], c[2]);
-      c[a] = t;
-      const double un = U[pn], kn = K[pn];
-      const double rn = inn ? LX[pn] : 0.0;
-      const double xn = a == 0 ? P[pn] : (a == 1 ? Q[pn] : U1[pn] - un);
-      const double ln = inn ? L[pn] : 0.0;
-      const double dX = side < 0 ? xp - xn : xn - xp;  // X_hi - X_lo
-      const double dL = side < 0 ? lp - ln : ln - lp;  // Lam~_hi - Lam~_lo
-      const double dU = side < 0 ? up - un : un - up;  // u_hi - u_lo
-      const double G = lam * (side < 0 ? rn - rp : rp - rn);
-      const double kf = 0.5 * (kp + kn);
+    for (int sd = 0; sd < 2; sd++) {
+      if (!has[a][sd]) continue;
+      const double unn = un[a][sd];
+      const double rnn = inn[a][sd] ? rn[a][sd] : 0.0;
+      const double xnn = a == 2 ? xn[a][sd] - unn : xn[a][sd];
+      const double lnn = inn[a][sd] ? ln[a][sd] : 0.0;
+      const double dX = sd == 0 ? xp - xnn : xnn - xp;  // X_hi - X_lo
+      const double dL = sd == 0 ? lp - lnn : lnn - lp;  // Lam~_hi - Lam~_lo
+      const double dU = sd == 0 ? up - unn : unn - up;  // u_hi - u_lo
+      const double G = lam * (sd == 0 ? rnn - rp : rp - rnn);
+      const double kf = 0.5 * (kp + kn[a][sd]);
       dk += 0.5 * (-lam * dX * dL + G * dU);
-      du += side < 0 ? G * kf : -(G * kf);
+      du += sd == 0 ? G * kf : -(G * kf);
     }
   }
-  UB[p] += du;
-  GK[p] += dk;
+  UB[p] = ub0 + du;
+  GK[p] = gk0 + dk;
 }
 
 // ubar = u - target; partial sums of ubar^2 per block
