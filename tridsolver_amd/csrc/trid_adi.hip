@@ -567,7 +567,9 @@ void tune_plan_t(tridAdiPlan_st *p, T *u, T *du) {
       if (!cc_rows_ok(n[dir], R, &cl)) continue;
       if (cc_build<T>(p, dir, R, &F, &cl) != TRID_STATUS_SUCCESS) continue;
       tables.push_back(F);
-      for (int pr = 0; pr <= (pair_ok && cc_pair_fits(R, cl) ? 1 : 0); pr++) {
+      // two lines per lane first: the usual winner, and the pick is the first
+      // candidate within 2 % of the fastest
+      for (int pr = (pair_ok && cc_pair_fits(R, cl) ? 1 : 0); pr >= 0; pr--) {
         ch.push_back({1, R, cl, 0, pr, p->cfg[dir], F});
         ch.push_back({1, R, cl, 1, pr, p->cfg[dir], F});
       }
