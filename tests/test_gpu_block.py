@@ -56,6 +56,39 @@ def test_block_every_lane_mapping(M, L):
     assert oracle.rel_err(exp, u.cpu().numpy()) <= 1e-12
 
 
+@pytest.mark.parametrize("M", [2, 3, 4, 8])
+def test_block_tuned_and_default_layouts(M):
+    """With autotuning on, the first call of a shape times every layout (the
+    tiled kernel and each lane count that divides M) and keeps the fastest;
+    the untuned default takes one lane per problem for M <= 3 and four for
+    M = 8 on batches of 65,536 problems or more.  Every layout solves as the
+    oracle does, forward and transposed (adjoint)."""
+    import tridsolver_amd as ta
+    import torch
+    for N, P in ((6, 3000), (3, 65536)):
+        A, B, C, d = blk_system(M, N, P, seed=20 + M)
+        exp = oracle.btsv(A, B, C, d, N, P, M, layout=0)
+        t = [dev(x) for x in (A, B, C, d)]
+        for tune in (0, 1):
+            ta.set_autotune(tune)
+            try:
+                u = blk.btsv(*t)
+                u2 = blk.btsv(*t)  # the tuned layout from the cache
+                g = torch.ones_like(u)
+                lam = blk.btsv_adjoint(t[0], t[1], t[2], u, g)[3]
+            finally:
+                ta.set_autotune(0)
+            err = oracle.rel_err(exp, u.cpu().numpy())
+            assert err <= 1e-12, f"M={M} N={N} P={P} tune={tune}: {err:.2e}"
+            assert torch.equal(u, u2)
+            # the transposed solve: lambda = A^{-T} 1 satisfies A^T lambda = 1;
+            # check sum_n,p <lambda, A d-direction> via the dot identity
+            # <A^{-1} d, 1> = <d, A^{-T} 1>
+            lhs = float(u.sum())
+            rhs = float((t[3] * lam).sum())
+            assert abs(lhs - rhs) <= 1e-10 * max(1.0, abs(lhs)), (M, N, P, tune, lhs, rhs)
+
+
 def test_block_reference_entry_point_and_scratch():
     """dbtsv_gpu with caller-provided Cstar/dstar (the reference interface);
     Cstar/dstar hold C'_n and d'_n of the forward sweep."""

@@ -23,13 +23,18 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/blktrid_hip.h"
 #include "../../include/trid_hip_ext.h"
 #include "trid_device.hpp"
+#include "trid_launch.hpp"
 #include "trid_runtime.hpp"
 
 namespace tridhip {
@@ -440,6 +445,7 @@ constexpr int kMaxGroups = 16;
 struct MixedArgs {
   BlkArgs<double> g[kMaxGroups];
   int M[kMaxGroups];
+  int L[kMaxGroups];  // lanes per problem (k_blk), or M for the tiled kernel
   long long first_block[kMaxGroups + 1];
   int ngroups;
 };
@@ -459,6 +465,18 @@ __global__ void __launch_bounds__(256) k_blk_mixed(MixedArgs a) {
   switch (a.M[gi]) {
 #define MIXED_CASE(MM)                                                            \
   case MM:                                                                        \
+    if constexpr (MM <= 3) {                                                      \
+      if (a.L[gi] == 1) {                                                         \
+        blk_block<double, MM, 1, 0>(a.g[gi], lb);                                 \
+        break;                                                                    \
+      }                                                                           \
+    }                                                                             \
+    if constexpr (MM == 8) {                                                      \
+      if (a.L[gi] == 4) {                                                         \
+        blk_block<double, MM, 4, 0>(a.g[gi], lb);                                 \
+        break;                                                                    \
+      }                                                                           \
+    }                                                                             \
     if constexpr (blk2_sizes(MM))                                                 \
       blk2_block<double, MM, 0>(a.g[gi], lb, smem);                               \
     else                                                                          \
@@ -508,14 +526,17 @@ namespace {
 
 std::atomic<int> g_force_lanes{0};
 
+// layout: kTiled = the tiled kernel (k_blk2, M = 3..6), L > 0 = k_blk with
+// L lanes per problem, 0 = the mixed launch's layout for M
+constexpr int kTiled = -1;
+
 template <typename T, int M, int TRANS>
 hipError_t launch_m(const BlkArgs<T> &g, int lanes, hipStream_t st) {
   constexpr int Ld = default_lanes(M);
-  const int L = lanes ? lanes : Ld;
+  const int L = lanes > 0 ? lanes : Ld;
   if constexpr (blk2_sizes(M)) {
-    // the default layout for these block sizes: one lane per block column,
-    // tiles through LDS
-    if (lanes == 0) {
+    // one lane per block column, tiles through LDS
+    if (lanes == 0 || lanes == kTiled) {
       const long long G = 4 * (kWave / M);
       const size_t lds = blk2_lds<T, M>();
       hipLaunchKernelGGL((k_blk2<T, M, TRANS>), dim3((unsigned)((g.P + G - 1) / G)), dim3(256),
@@ -572,6 +593,60 @@ tridStatus_t check_blk(const void *A, const void *B, const void *C, const void *
   return TRID_STATUS_SUCCESS;
 }
 
+// Layout of a batch without tuning, from sweeps on MI355X at N = 128,
+// P = 65,536 (ms, FP64): one lane per problem wins for M <= 3 once the
+// batch fills the chip with it (M = 2: 0.350 against 0.369 with two lanes;
+// M = 3: 0.919 against 0.989 tiled, 1.058 with three lanes), four lanes for
+// M = 8 (5.08 against 5.72 with eight, 6.36 with two); the tiled kernel for
+// M = 4, 5 (1.34 against 1.46 / 1.63 / 2.13 with 4 / 2 / 1 lanes; 2.26
+// against 2.67 / 3.76).  Smaller batches keep more lanes per problem.
+int default_layout(int M, long long P) {
+  if (P >= 65536) {
+    if (M <= 3) return 1;
+    if (M == 8) return 4;
+  }
+  return blk2_sizes(M) ? kTiled : default_lanes(M);
+}
+
+std::mutex g_blk_mu;
+// tuned layout per (precision, transposed, M, N, P)
+std::map<std::tuple<int, int, int, int, int>, int> g_blk_tuned;
+
+// The layout of a call: forced lanes, else the tuned layout (tuned at the
+// first call of a shape with autotuning on and the stream not capturing: the
+// call's own arrays, which the solve only reads, or only overwrites, are
+// used for the timing), else the default.
+template <typename T, int TRANS>
+int choose_layout(const BlkArgs<T> &g, int M, hipStream_t st) {
+  const int forced = g_force_lanes.load();
+  if (forced) return forced;
+  const int def = default_layout(M, g.P);
+  if (autotune_mode() != 1) return def;
+  const auto key = std::make_tuple((int)sizeof(T), TRANS, M, g.N, g.P);
+  {
+    std::lock_guard<std::mutex> lk(g_blk_mu);
+    auto it = g_blk_tuned.find(key);
+    if (it != g_blk_tuned.end()) return it->second;
+  }
+  if (capturing(st)) return def;
+  std::vector<int> cands{def};
+  if (blk2_sizes(M) && def != kTiled) cands.push_back(kTiled);
+  for (int L = 1; L <= M && L <= 8; L++)
+    if (M % L == 0 && L != def) cands.push_back(L);
+  std::vector<float> tm;
+  auto run = [&](int i) { return launch_blk<T, TRANS>(g, M, cands[i], st); };
+  const int i = pick_fastest((int)cands.size(), [](int) { return true; }, run, st, &tm, 3, 2);
+  const int pick = i >= 0 ? cands[i] : def;
+  const char *log = std::getenv("TRIDHIP_TUNE_LOG");
+  if (log && log[0] == '1')
+    for (size_t c = 0; c < cands.size(); c++)
+      fprintf(stderr, "[tridhip tune] block M=%d N=%d P=%d %s layout %d: %.4f ms\n", M, g.N, g.P,
+              TRANS ? "transposed" : "", cands[c], tm[c]);
+  std::lock_guard<std::mutex> lk(g_blk_mu);
+  g_blk_tuned[key] = pick;
+  return pick;
+}
+
 template <typename T>
 tridStatus_t btsv(const T *A, const T *B, const T *C, const T *d, T *Cs, T *ds,
                   T *u, int N, int P, int M, int sync) {
@@ -588,7 +663,8 @@ tridStatus_t btsv(const T *A, const T *B, const T *C, const T *d, T *Cs, T *ds,
     ds = w + mat;
   }
   BlkArgs<T> g{A, B, C, d, Cs, ds, u, N, P};
-  s = check_hip(launch_blk<T, 0>(g, M, g_force_lanes.load(), st), "block kernel launch");
+  s = check_hip(launch_blk<T, 0>(g, M, choose_layout<T, 0>(g, M, st), st),
+                "block kernel launch");
   if (s) return s;
   if (sync) return check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
   return TRID_STATUS_SUCCESS;
@@ -644,7 +720,10 @@ EXTERN_C tridStatus_t tridDbtsvMixedBatch(const tridDbtsvGroup_t *groups,
     a.g[k] = BlkArgs<double>{G.A, G.B, G.C, G.d, G.Cstar, G.dstar, G.u, G.N, G.P};
     a.M[k] = G.M;
     a.first_block[k] = nb;
-    const int L = default_lanes(G.M);
+    // the untuned layout of a separate launch where the mixed kernel has it
+    const int dl = default_layout(G.M, G.P);
+    const int L = (dl == 1 && G.M <= 3) || (dl == 4 && G.M == 8) ? dl : default_lanes(G.M);
+    a.L[k] = L;
     const long long per = 4LL * (kWave / L);
     nb += (G.P + per - 1) / per;
     k++;
@@ -686,7 +765,8 @@ EXTERN_C tridStatus_t tridDbtsvAdjoint(const double *A, const double *B,
   if (!w) return TRID_STATUS_ALLOC_FAILED;  // reason recorded by workspace()
   // lambda = A^{-T} g  (written straight into grad_d)
   BlkArgs<double> ga{A, B, C, g, w, w + mat, grad_d, N, P};
-  s = check_hip(launch_blk<double, 1>(ga, M, g_force_lanes.load(), st), "adjoint kernel launch");
+  s = check_hip(launch_blk<double, 1>(ga, M, choose_layout<double, 1>(ga, M, st), st),
+                "adjoint kernel launch");
   if (s) return s;
   const long long total = (long long)N * P * M * M;
   hipLaunchKernelGGL(k_blk_grads<double>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
