@@ -197,6 +197,194 @@ __device__ __forceinline__ void blk_problem(const BlkArgs<T> &g, long long p,
   }
 }
 
+// Prefetching form of blk_problem (used where blk_prefetch says): row n+1's
+// blocks are loaded
+// while row n is eliminated, and row n-1's C'/d' while row n is
+// substituted.  With one lane per problem a batch of 65,536 problems fills
+// one wave per SIMD, so without this every block row waits a full memory
+// latency.  Same operations in the same order as blk_problem.
+template <typename T, int M, int L, int TRANS>
+__device__ __forceinline__ void blk_problem_pf(const BlkArgs<T> &g, long long p,
+                                               bool active, int j, int gbase) {
+  constexpr int CM = M / L;
+  const long long P = g.P;
+  const int N = g.N;
+  auto blk = [&](int n) { return ((long long)n * P + p) * (M * M); };
+  auto vec = [&](int n) { return ((long long)n * P + p) * M; };
+  auto sub = [&](int n, int i, int k) -> T {
+    return TRANS ? g.C[blk(n - 1) + k * M + i] : g.A[blk(n) + i * M + k];
+  };
+  auto diag = [&](int n, int i, int c) -> T {
+    return TRANS ? g.B[blk(n) + c * M + i] : g.B[blk(n) + i * M + c];
+  };
+  auto sup = [&](int n, int i, int c) -> T {
+    return TRANS ? g.A[blk(n + 1) + c * M + i] : g.C[blk(n) + i * M + c];
+  };
+  // the inputs of a block row: diagonal and super-diagonal columns of this
+  // lane, the right-hand side and the whole sub-diagonal block
+  struct RowIn {
+    T X[M][CM], C[M][CM], v[M], A[M][M];
+  };
+  auto load_row = [&](int n, RowIn &r) {
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+#pragma unroll
+      for (int t = 0; t < CM; t++) {
+        const int c = j + L * t;
+        r.X[i][t] = diag(n, i, c);
+        r.C[i][t] = n < N - 1 ? sup(n, i, c) : T(0);
+      }
+      r.v[i] = g.d[vec(n) + i];
+#pragma unroll
+      for (int k = 0; k < M; k++) r.A[i][k] = n > 0 ? sub(n, i, k) : T(0);
+    }
+  };
+
+  T Cp[M][CM], dp[M];
+#pragma unroll
+  for (int i = 0; i < M; i++) {
+    dp[i] = T(0);
+#pragma unroll
+    for (int t = 0; t < CM; t++) Cp[i][t] = T(0);
+  }
+  RowIn cur;
+  if (active) load_row(0, cur);
+  for (int n = 0; n < N; n++) {
+    RowIn nxt;
+    if (active && n + 1 < N) load_row(n + 1, nxt);
+    T X[M][CM], Cn[M][CM], v[M];
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+#pragma unroll
+        for (int t = 0; t < CM; t++) {
+          X[i][t] = cur.X[i][t];
+          Cn[i][t] = cur.C[i][t];
+        }
+        v[i] = cur.v[i];
+      }
+      if (n > 0) {
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+#pragma unroll
+          for (int i = 0; i < M; i++) {
+#pragma unroll
+            for (int t = 0; t < CM; t++) X[i][t] -= cur.A[i][k] * Cp[k][t];
+            v[i] -= cur.A[i][k] * dp[k];
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+        v[i] = T(0);
+#pragma unroll
+        for (int t = 0; t < CM; t++) {
+          X[i][t] = (j + L * t == i) ? T(1) : T(0);
+          Cn[i][t] = T(0);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < M; q++) {
+      T col[M];
+      const int tq = q / L;
+#pragma unroll
+      for (int i = 0; i < M; i++)
+        col[i] = (L == 1) ? X[i][tq] : lane_read(X[i][tq], min(gbase + (q % L), kWave - 1));
+      const T inv = rcp(col[q]);
+#pragma unroll
+      for (int t = 0; t < CM; t++) {
+        X[q][t] *= inv;
+        Cn[q][t] *= inv;
+      }
+      v[q] *= inv;
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+        if (i == q) continue;
+        const T f = col[i];
+#pragma unroll
+        for (int t = 0; t < CM; t++) {
+          X[i][t] -= f * X[q][t];
+          Cn[i][t] -= f * Cn[q][t];
+        }
+        v[i] -= f * v[q];
+      }
+    }
+    if (active) {
+#pragma unroll
+      for (int t = 0; t < CM; t++) {
+        const int c = j + L * t;
+        if (n < N - 1) {
+#pragma unroll
+          for (int i = 0; i < M; i++) g.Cs[blk(n) + i * M + c] = Cn[i][t];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < M; i++)
+        if (i % L == j) g.ds[vec(n) + i] = v[i];
+    }
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+      dp[i] = v[i];
+#pragma unroll
+      for (int t = 0; t < CM; t++) Cp[i][t] = Cn[i][t];
+    }
+    cur = nxt;
+  }
+
+  // backward, with row n-1's C'_n-1 and d'_n-1 loaded during row n
+  T un[M];
+#pragma unroll
+  for (int i = 0; i < M; i++) un[i] = dp[i];
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < M; i++)
+      if (i % L == j) g.u[vec(N - 1) + i] = un[i];
+  }
+  if (!active) return;
+  T bC[M][M], bd[M];
+  auto load_back = [&](int n) {
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+      bd[i] = g.ds[vec(n) + i];
+#pragma unroll
+      for (int k = 0; k < M; k++) bC[i][k] = g.Cs[blk(n) + i * M + k];
+    }
+  };
+  if (N >= 2) load_back(N - 2);
+  for (int n = N - 2; n >= 0; n--) {
+    T cC[M][M], cd[M];
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+      cd[i] = bd[i];
+#pragma unroll
+      for (int k = 0; k < M; k++) cC[i][k] = bC[i][k];
+    }
+    if (n >= 1) load_back(n - 1);
+    T nu[M];
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+      T acc = cd[i];
+#pragma unroll
+      for (int k = 0; k < M; k++) acc -= cC[i][k] * un[k];
+      nu[i] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+      un[i] = nu[i];
+      if (i % L == j) g.u[vec(n) + i] = un[i];
+    }
+  }
+}
+
+// Where blk_problem_pf is used: measured on MI355X (N = 128, P = 65,536) it
+// helps M = 3 with one lane per problem (0.913 -> 0.870 ms) and costs the
+// layouts with more waves per problem set or smaller rows (M = 2, one lane:
+// 0.350 -> 0.368; M = 3, three lanes: 1.058 -> 1.091; M = 4, four lanes:
+// 1.46 -> 1.49).
+__host__ __device__ constexpr bool blk_prefetch(int M, int L) { return M == 3 && L == 1; }
+
 // one work-group of 256 threads = 4 waves; each wave holds 64/L problems
 template <typename T, int M, int L, int TRANS>
 __device__ __forceinline__ void blk_block(const BlkArgs<T> &g, long long block) {
@@ -208,15 +396,24 @@ __device__ __forceinline__ void blk_block(const BlkArgs<T> &g, long long block) 
   const int gbase = grp * L;
   const long long p = (block * 4 + wave) * PW + grp;
   const bool active = grp < PW && p < g.P;
-  blk_problem<T, M, L, TRANS>(g, p, active, j, gbase);
+  if constexpr (blk_prefetch(M, L))
+    blk_problem_pf<T, M, L, TRANS>(g, p, active, j, gbase);
+  else
+    blk_problem<T, M, L, TRANS>(g, p, active, j, gbase);
 }
 
 // Minimum waves per SIMD requested from the register allocator (it would
 // otherwise hoist every block load of a row and spill for M >= 6).
 constexpr int blk_min_waves(int M) { return M <= 2 ? 8 : (M <= 4 ? 4 : 2); }
 
+// The prefetching form holds two rows' inputs: fewer waves per SIMD (it runs
+// with one lane per problem, where a batch fills few waves anyway).
+constexpr int blk_waves(int M, int L) {
+  return blk_prefetch(M, L) ? 2 : blk_min_waves(M);
+}
+
 template <typename T, int M, int L, int TRANS>
-__global__ void __launch_bounds__(256, blk_min_waves(M)) k_blk(BlkArgs<T> g) {
+__global__ void __launch_bounds__(256, blk_waves(M, L)) k_blk(BlkArgs<T> g) {
   blk_block<T, M, L, TRANS>(g, blockIdx.x);
 }
 
