@@ -30,10 +30,11 @@ EXTERN_C tridStatus_t tridSetAutotune(int mode);
  * start / stop (hipEvent_t, either may be NULL) around its own execution
  * (hipExtLaunchKernel timestamps the dispatch packet), so that
  * hipEventElapsedTime(start, stop) is the kernel's duration without the
- * dispatch latency an event recorded before the call adds.  One launch
- * consumes them.  The kernels of strided solves (not split-line) and of the
- * ADI stages honour them; the fused ADI x stage, its y and z sweeps each
- * launch one kernel per tridAdiStage call. */
+ * dispatch latency an event recorded before the call adds.  The next
+ * strided solve or ADI stage consumes them: a single-kernel strided solve and
+ * each tridAdiStage call (fused x stage, y sweep, z sweep: one kernel each)
+ * record them; a split-line or one-thread-per-system solve discards them.
+ * Other calls (ragged, block, adjoint) leave them pending. */
 EXTERN_C tridStatus_t tridSetLaunchEvents(void *start, void *stop);
 
 /* Short text of the last failing call on this thread ("" if none). */

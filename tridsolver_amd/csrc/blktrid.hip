@@ -812,7 +812,8 @@ std::map<std::tuple<int, int, int, int, int>, int> g_blk_tuned;
 // The layout of a call: forced lanes, else the tuned layout (tuned at the
 // first call of a shape with autotuning on and the stream not capturing: the
 // call's own arrays, which the solve only reads, or only overwrites, are
-// used for the timing), else the default.
+// used for the timing; a call whose u is its d is not tuned), else the
+// default.
 template <typename T, int TRANS>
 int choose_layout(const BlkArgs<T> &g, int M, hipStream_t st) {
   const int forced = g_force_lanes.load();
@@ -825,7 +826,10 @@ int choose_layout(const BlkArgs<T> &g, int M, hipStream_t st) {
     auto it = g_blk_tuned.find(key);
     if (it != g_blk_tuned.end()) return it->second;
   }
-  if (capturing(st)) return def;
+  // (timing runs the solve several times: not when its output overwrites
+  // its right-hand side, and not inside a caller's capture)
+  if (capturing(st) || static_cast<const void *>(g.u) == static_cast<const void *>(g.d))
+    return def;
   std::vector<int> cands{def};
   if (blk2_sizes(M) && def != kTiled) cands.push_back(kTiled);
   for (int L = 1; L <= M && L <= 8; L++)

@@ -532,6 +532,8 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
     return fail(TRID_STATUS_INVALID_VALUE, "split-line solve: too many systems");
   cfg.idx32 = idx32;
   set_last_config(cfg_name(cfg, sizeof(T)));
+  // launch events (tridSetLaunchEvents) time single-kernel solves only
+  if (cfg.kind != 1) (void)take_launch_events();
   tridStatus_t s = run_cfg<T, INC>(cfg, a, b, c, d, u, D, st, fixed, fixed_bytes);
   if (s != TRID_STATUS_SUCCESS) return s;
   if (sync) return check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
