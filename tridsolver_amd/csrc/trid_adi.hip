@@ -388,9 +388,8 @@ tridStatus_t stage_t(tridAdiPlan_st *p, T *u, T *du, int stage, hipStream_t st) 
     e = launch_fused<T>(p, u, du, st);
   } else if (stage == 0) {
     dim3 grid((p->nx + 63) / 64, (p->ny + 3) / 4, (p->nz + kRhsPlanes - 1) / kRhsPlanes);
-    hipLaunchKernelGGL(k_adi_rhs<T>, grid, dim3(64, 4), 0, st, u, du, p->nx, p->ny,
-                       p->nz, (T)p->lambda);
-    e = hipGetLastError();
+    e = launch_kernel(k_adi_rhs<T>, grid, dim3(64, 4), 0u, st, (const T *)u, du, p->nx, p->ny,
+                      p->nz, (T)p->lambda);
   } else if (stage == 1) {
     const PartCfg &c = aligned16(du) ? p->cfg[0] : p->cfg_x_unaligned;
     e = launch_part<T, 0>(c, cf, du, du, p->desc[0], st);
