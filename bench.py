@@ -90,20 +90,23 @@ def kernel_of_config(cfg: str, dtype: str, inc: int, coef: str):
     library (tridGetLastConfig, an ADI plan's stage), as rocprofv3 names it."""
     import re
     T = "double" if dtype == "f64" else "float"
+    # the last template argument is the cache policy (0 non-temporal, 1 ordinary)
     m = re.search(r"cc<R(\d+),cps\d+((?:,\w+)*)>", cfg)
     if m:
-        ns = 2 if "pair" in m.group(2).split(",") else 1
-        return f"k_cc_sweep<{T}, {inc}, {m.group(1)}, unsigned int, {ns}>"
+        fl = m.group(2).split(",")
+        ns = 2 if "pair" in fl else 1
+        return f"k_cc_sweep<{T}, {inc}, {m.group(1)}, unsigned int, {ns}, {int('cached' in fl)}>"
     m = re.search(r"part<\w+,o(\d),R(\d+),cps(\d+),spb\d+((?:,\w+)*)>", cfg)
     if m:
         fl = m.group(4).split(",")
         staged = "staged" in fl
         vec = 4 if "gstaged" in fl else (3 if staged and int(m.group(3)) > 64 else
                                          (2 if staged else (1 if "vec" in fl else 0)))
-        return f"k_part<{T}, {inc}, {m.group(1)}, {m.group(2)}, {vec}, unsigned int, {coef}<{T}> >"
-    m = re.search(r"rhs\+x\(fused,R(\d+)\)", cfg)
+        return (f"k_part<{T}, {inc}, {m.group(1)}, {m.group(2)}, {vec}, unsigned int, "
+                f"{coef}<{T}>, {int('cached' in fl)}>")
+    m = re.search(r"rhs\+x\(fused,R(\d+)((?:,\w+)*)\)", cfg)
     if m:
-        return f"k_adi_rhs_x<{T}, {m.group(1)}>"
+        return f"k_adi_rhs_x<{T}, {m.group(1)}, {int('cached' in m.group(2).split(','))}>"
     return None
 
 

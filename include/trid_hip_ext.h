@@ -67,6 +67,19 @@ EXTERN_C tridStatus_t tridForceConfig(int kind, int rows, int cps, int spb);
  * restores the automatic choice. */
 EXTERN_C tridStatus_t tridForceWideIndex(int on);
 
+/* Cache policy of the solvers' streaming loads and stores (the coefficient,
+ * right-hand-side and solution arrays): policy 0 = non-temporal (they do not
+ * allocate in L2 or the 256 MB last-level cache; best when the arrays exceed
+ * that cache), 1 = ordinary (arrays that fit stay in the last-level cache,
+ * so the next kernel or call reads them from there), -1 (the default) =
+ * automatic: ordinary when the arrays of the call -- of an ADI plan: u and
+ * du -- span at most 384 MiB, else non-temporal.  Applies to later strided
+ * solves and to ADI plans created afterwards; both policies give bitwise
+ * equal results.  Solves with 64-bit element offsets are always
+ * non-temporal.  No counterpart in the reference (its kernels have no cache
+ * hints). */
+EXTERN_C tridStatus_t tridForceCachePolicy(int policy);
+
 /* Diagnostic: y[i] = the solvers' reciprocal of x[i] for n device values
  * (double when is_double, else float).  The solvers form every pivot
  * reciprocal with this routine; tests check its error bound. */

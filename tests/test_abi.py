@@ -66,6 +66,15 @@ def test_library_loads_without_gpu():
     assert st == 3
 
 
+def test_cache_policy_switch_validates():
+    import tridsolver_amd as ta
+    for bad in (2, -2):
+        with pytest.raises(ta.TridError):
+            ta.force_cache_policy(bad)
+    for ok in (0, 1, -1):
+        ta.force_cache_policy(ok)
+
+
 def test_status_values_match_reference_enum():
     # scalar/include/trid_common.h:65-76
     import tridsolver_amd

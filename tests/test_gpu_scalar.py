@@ -235,7 +235,8 @@ def test_default_strided_layout(dtype, N, cfg, sd):
     side)."""
     dims = [64, N, 3] if sd == 1 else [64, 3, N]
     run_case(dims, sd, dtype, False, seed=N)
-    assert ta.last_config().endswith(cfg), ta.last_config()
+    # (the cache policy, ",cached" for these small batches, is not the layout)
+    assert ta.last_config().replace(",cached", "").endswith(cfg), ta.last_config()
 
 
 @pytest.mark.parametrize("R", [4, 8, 16])

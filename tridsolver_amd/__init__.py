@@ -83,6 +83,14 @@ def force_wide_index(on: bool) -> None:
     _check(lib().tridForceWideIndex(1 if on else 0), "tridForceWideIndex")
 
 
+def force_cache_policy(policy: int) -> None:
+    """Cache policy of the solvers' streaming loads and stores: 0 non-temporal,
+    1 ordinary (arrays that fit the 256 MB last-level cache stay there for
+    the next kernel or call), -1 automatic (ordinary up to 384 MiB of arrays
+    per call or ADI step).  Both give bitwise equal results."""
+    _check(lib().tridForceCachePolicy(int(policy)), "tridForceCachePolicy")
+
+
 def last_config() -> str:
     return lib().tridGetLastConfig().decode()
 

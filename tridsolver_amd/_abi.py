@@ -31,6 +31,7 @@ SIGNATURES = {
     "tridGetLastConfig": (C.c_char_p, []),
     "tridForceConfig": (ST, [I, I, I, I]),
     "tridForceWideIndex": (ST, [I]),
+    "tridForceCachePolicy": (ST, [I]),
     "tridDebugReciprocal": (ST, [P, P, C.c_longlong, I]),
     "tridDmtsvStridedBatchNoAlloc": (ST, [P, P, P, P, P, I, I, IP, IP, IP, I]),
     # trid_adi.h

@@ -34,7 +34,7 @@ constexpr int kRhsPlanes = 8;  // z planes per thread of the RHS kernel
 // registers.  The sum is formed in the reference's order
 // (apps/adi/src/adi_orig.c:183-186) without contraction, so it matches the
 // reference's rounding.
-template <typename T>
+template <typename T, int CP>
 __global__ void __launch_bounds__(256)
     k_adi_rhs(const T *__restrict__ u, T *__restrict__ du, int nx, int ny,
               int nz, T lam) {
@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(256)
       s = s - six;
       v = lam * s;
     }
-    st_nt(du + idx, v);
+    st_stream<CP>(du + idx, v);
     um = uc;
     uc = up;
     idx += nxy;
@@ -96,7 +96,7 @@ __device__ __forceinline__ T rhs_point(T l, T r, T ym, T yp, T zm, T zp, T c,
 // grid rounded down to a multiple of 8), so that each XCD -- work-groups are
 // dealt to the 8 XCDs round-robin -- sweeps one contiguous range of planes
 // and finds the neighbour planes of its lines in its own L2.
-template <typename T, int R>
+template <typename T, int R, int CP>
 __global__ void __launch_bounds__(256)
     k_adi_rhs_x(const T *__restrict__ u, T *__restrict__ du, int nx, int ny,
                 int nz, T lam, CoefConst<T> cf, int cps_log2, int xcd) {
@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(256)
   reduced_wave<T>(f, l, k, cps, lane, xf, xl);
   T y[R];
   chunk_expand<T, R>(a, c, d, xf, xl, y);
-  if (wave_active) store_chunk_staged<T, R>(du + wo, stage, lane, y, false);
+  if (wave_active) store_chunk_staged<T, R, CP>(du + wo, stage, lane, y, false);
 }
 
 namespace {
@@ -219,6 +219,7 @@ struct tridAdiPlan_st {
   int fused_R = 0;          // >0: fused RHS + x sweep with this chunk length
   int fused_cps_log2 = 0;
   int fused_xcd = 0;        // XCD-contiguous block order of the fused kernel
+  int cp = 0;               // cache policy of every stage's streaming accesses
   // y and z sweeps with precomputed factors (trid_cc.hpp): chunk length (0 =
   // the generic kernel), chunks per system and the factor table
   int cc_R[3] = {0, 0, 0};
@@ -254,18 +255,21 @@ hipError_t launch_fused(tridAdiPlan_st *p, T *u, T *du, hipStream_t st) {
   const long long nlines = (long long)p->ny * p->nz;
   const unsigned blocks = (unsigned)((nlines + lines_per_block - 1) / lines_per_block);
   const size_t lds = 256 * (size_t)p->fused_R * sizeof(T);
-  if (p->fused_R == 8)
-    (void)launch_kernel(k_adi_rhs_x<T, 8>, dim3(blocks), dim3(256), (std::uint32_t)lds, st, u, du,
-                       p->nx, p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2,
-                       p->fused_xcd);
-  else if (p->fused_R == 4)
-    (void)launch_kernel(k_adi_rhs_x<T, 4>, dim3(blocks), dim3(256), (std::uint32_t)lds, st, u, du,
-                       p->nx, p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2,
-                       p->fused_xcd);
-  else
-    (void)launch_kernel(k_adi_rhs_x<T, 16>, dim3(blocks), dim3(256), (std::uint32_t)lds, st, u, du,
-                       p->nx, p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2,
-                       p->fused_xcd);
+#define TRID_FUSED(RR, CC)                                                                   \
+  (void)launch_kernel(k_adi_rhs_x<T, RR, CC>, dim3(blocks), dim3(256), (std::uint32_t)lds, st, \
+                      u, du, p->nx, p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2,       \
+                      p->fused_xcd)
+#define TRID_FUSED_R(CC)                  \
+  if (p->fused_R == 8) TRID_FUSED(8, CC);  \
+  else if (p->fused_R == 4) TRID_FUSED(4, CC); \
+  else TRID_FUSED(16, CC);
+  if (p->cp == kCached) {
+    TRID_FUSED_R(kCached)
+  } else {
+    TRID_FUSED_R(kStream)
+  }
+#undef TRID_FUSED_R
+#undef TRID_FUSED
   return hipGetLastError();
 }
 
@@ -333,7 +337,7 @@ bool cc_pair_ok(const SysDesc &D, const T *d, const T *u, int inc) {
 
 template <typename T, int INC>
 hipError_t launch_cc_t(int R, int cps_log2, int xcd, int pair, const void *F, const SysDesc &D,
-                       int idx32, T *d, T *u, hipStream_t st) {
+                       int idx32, int cp, T *d, T *u, hipStream_t st) {
   const int cps = 1 << cps_log2;
   const int ns = pair ? 2 : 1;
   const long long per = (long long)kWave * ns;
@@ -341,29 +345,32 @@ hipError_t launch_cc_t(int R, int cps_log2, int xcd, int pair, const void *F, co
   const size_t lds = 2 * (size_t)ns * kWave * cps * sizeof(T);
   const dim3 grid(blocks), block(kWave * cps);
   const T *f = static_cast<const T *>(F);
-#define TRID_CC(RR, II, NN)                                                               \
-  (void)launch_kernel(k_cc_sweep<T, INC, RR, II, NN>, grid, block, (std::uint32_t)lds, st, f, \
-                      d, u, D, cps_log2, xcd)
-#define TRID_CC_R(II, NN)      \
-  if (R == 8) TRID_CC(8, II, NN);       \
-  else if (R == 16) TRID_CC(16, II, NN); \
-  else TRID_CC(32, II, NN);
-  // pairs: chunk lengths 8, 16 and 32 (cc_pair_fits)
-  if (idx32) {
+#define TRID_CC(RR, II, NN, CC)                                                           \
+  (void)launch_kernel(k_cc_sweep<T, INC, RR, II, NN, CC>, grid, block, (std::uint32_t)lds, st, \
+                      f, d, u, D, cps_log2, xcd)
+#define TRID_CC_R(II, NN, CC)             \
+  if (R == 8) TRID_CC(8, II, NN, CC);       \
+  else if (R == 16) TRID_CC(16, II, NN, CC); \
+  else TRID_CC(32, II, NN, CC);
+  // pairs: chunk lengths 8, 16 and 32 (cc_pair_fits); the cached policy
+  // with 32-bit offsets only
+  if (idx32 && cp == kCached) {
     if (pair) {
-      if (R == 8) TRID_CC(8, uint32_t, 2);
-      else if (R == 16) TRID_CC(16, uint32_t, 2);
-      else TRID_CC(32, uint32_t, 2);
+      TRID_CC_R(uint32_t, 2, kCached)
     } else {
-      TRID_CC_R(uint32_t, 1)
+      TRID_CC_R(uint32_t, 1, kCached)
+    }
+  } else if (idx32) {
+    if (pair) {
+      TRID_CC_R(uint32_t, 2, kStream)
+    } else {
+      TRID_CC_R(uint32_t, 1, kStream)
     }
   } else {
     if (pair) {
-      if (R == 8) TRID_CC(8, long long, 2);
-      else if (R == 16) TRID_CC(16, long long, 2);
-      else TRID_CC(32, long long, 2);
+      TRID_CC_R(long long, 2, kStream)
     } else {
-      TRID_CC_R(long long, 1)
+      TRID_CC_R(long long, 1, kStream)
     }
   }
 #undef TRID_CC_R
@@ -376,7 +383,7 @@ hipError_t launch_cc(const tridAdiPlan_st *p, int dir, T *d, T *u, hipStream_t s
   const int pair = p->cc_pair[dir] && cc_pair_fits(p->cc_R[dir], p->cc_cps_log2[dir]) &&
                    cc_pair_ok<T>(p->desc[dir], d, u, INC);
   return launch_cc_t<T, INC>(p->cc_R[dir], p->cc_cps_log2[dir], p->cc_xcd[dir], pair,
-                             p->cc_F[dir], p->desc[dir], p->cfg[dir].idx32, d, u, st);
+                             p->cc_F[dir], p->desc[dir], p->cfg[dir].idx32, p->cp, d, u, st);
 }
 
 // Stage 4 = fused RHS + x sweep (plans with fused_R > 0 only).
@@ -388,17 +395,26 @@ tridStatus_t stage_t(tridAdiPlan_st *p, T *u, T *du, int stage, hipStream_t st) 
     e = launch_fused<T>(p, u, du, st);
   } else if (stage == 0) {
     dim3 grid((p->nx + 63) / 64, (p->ny + 3) / 4, (p->nz + kRhsPlanes - 1) / kRhsPlanes);
-    e = launch_kernel(k_adi_rhs<T>, grid, dim3(64, 4), 0u, st, (const T *)u, du, p->nx, p->ny,
-                      p->nz, (T)p->lambda);
+    if (p->cp == kCached)
+      e = launch_kernel(k_adi_rhs<T, kCached>, grid, dim3(64, 4), 0u, st, (const T *)u, du,
+                        p->nx, p->ny, p->nz, (T)p->lambda);
+    else
+      e = launch_kernel(k_adi_rhs<T, kStream>, grid, dim3(64, 4), 0u, st, (const T *)u, du,
+                        p->nx, p->ny, p->nz, (T)p->lambda);
   } else if (stage == 1) {
-    const PartCfg &c = aligned16(du) ? p->cfg[0] : p->cfg_x_unaligned;
+    PartCfg c = aligned16(du) ? p->cfg[0] : p->cfg_x_unaligned;
+    c.cp = p->cp;
     e = launch_part<T, 0>(c, cf, du, du, p->desc[0], st);
   } else if (stage == 2) {
+    PartCfg c = p->cfg[1];
+    c.cp = p->cp;
     e = p->cc_R[1] ? launch_cc<T, 0>(p, 1, du, du, st)
-                   : launch_part<T, 0>(p->cfg[1], cf, du, du, p->desc[1], st);
+                   : launch_part<T, 0>(c, cf, du, du, p->desc[1], st);
   } else {
+    PartCfg c = p->cfg[2];
+    c.cp = p->cp;
     e = p->cc_R[2] ? launch_cc<T, 1>(p, 2, du, u, st)
-                   : launch_part<T, 1>(p->cfg[2], cf, du, u, p->desc[2], st);
+                   : launch_part<T, 1>(c, cf, du, u, p->desc[2], st);
   }
   return check_hip(e, "ADI kernel launch");
 }
@@ -644,6 +660,27 @@ void tune_plan_t(tridAdiPlan_st *p, T *u, T *du) {
     set_fused(p, ch[i >= 0 ? i : 0].first);
     p->fused_xcd = ch[i >= 0 ? i : 0].second;
   }
+  // cache policy of the whole step (the other policy changes what every
+  // stage finds in the last-level cache, so whole steps are timed): the
+  // footprint rule's choice first, kept unless the other is more than 2 %
+  // faster
+  if (p->cfg[0].idx32 && forced_cache_policy() < 0) {
+    const int pol[2] = {p->cp, p->cp == kCached ? kStream : kCached};
+    auto cand = [&](int i) {
+      p->cp = pol[i];
+      x_stage();
+      stage(2);
+      stage(3);
+      return hipSuccess;
+    };
+    std::vector<float> tm;
+    const int i = pick_in_step(2, []() {}, cand, []() {}, st, &tm);
+    if (tune_log())
+      for (int c = 0; c < 2; c++)
+        fprintf(stderr, "[tridhip tune] step, cache policy %s: %.4f ms\n",
+                pol[c] == kCached ? "cached" : "stream", tm[c]);
+    p->cp = pol[i >= 0 ? i : 0];
+  }
   (void)hipMemcpyAsync(u, save, bytes, hipMemcpyDeviceToDevice, st);
   (void)hipMemcpyAsync(du, save + npts, bytes, hipMemcpyDeviceToDevice, st);
   (void)hipStreamSynchronize(st);
@@ -653,18 +690,24 @@ void tune_plan_t(tridAdiPlan_st *p, T *u, T *du) {
 
 std::string plan_text(const tridAdiPlan_st *p) {
   const size_t ts = p->dbl ? 8 : 4;
+  const std::string cached = p->cp == kCached ? ",cached" : "";
+  auto part_name = [&](int dir) {
+    PartCfg c = p->cfg[dir];
+    c.cp = p->cp;
+    return cfg_name(c, ts);
+  };
   auto sweep_name = [&](int dir) {
-    if (!p->cc_R[dir]) return cfg_name(p->cfg[dir], ts);
+    if (!p->cc_R[dir]) return part_name(dir);
     return "cc<R" + std::to_string(p->cc_R[dir]) + ",cps" +
            std::to_string(1 << p->cc_cps_log2[dir]) + (p->cc_xcd[dir] ? ",xcd" : "") +
-           (p->cc_pair[dir] ? ",pair>" : ">");
+           (p->cc_pair[dir] ? ",pair" : "") + cached + ">";
   };
-  std::string fx = "rhs<";
+  std::string fx = "rhs" + cached + "<";
   if (p->fused_R)
-    fx = "rhs+x(fused,R" + std::to_string(p->fused_R) + (p->fused_xcd ? ",xcd" : "") + ")<";
-  return fx +
-         (p->dbl ? "double" : "float") + "> x:" + cfg_name(p->cfg[0], ts) +
-         " y:" + sweep_name(1) + " z:" + sweep_name(2);
+    fx = "rhs+x(fused,R" + std::to_string(p->fused_R) + (p->fused_xcd ? ",xcd" : "") + cached +
+         ")<";
+  return fx + (p->dbl ? "double" : "float") + "> x:" + part_name(0) + " y:" + sweep_name(1) +
+         " z:" + sweep_name(2);
 }
 
 // With autotuning on, a plan tunes at its first use, on that call's arrays,
@@ -711,6 +754,8 @@ EXTERN_C tridStatus_t tridAdiPlanCreate(tridAdiPlan_t *plan, int nx, int ny,
   p->mode = mode;
   const size_t ts = p->dbl ? 8 : 4;
   const int idx32 = (unsigned long long)nx * ny * nz * ts < (1ULL << 32) && !wide_index();
+  // cache policy from the step's working set, u and du
+  p->cp = idx32 ? cache_policy(2 * (size_t)nx * ny * nz * ts) : kStream;
   const int n[3] = {nx, ny, nz};
   for (int dir = 0; dir < 3; dir++) {
     p->desc[dir] = grid_desc(nx, ny, nz, dir);

@@ -159,8 +159,9 @@ __global__ void __launch_bounds__(kWave)
 // systems' rows as one 16-byte access (the host requires adjacent systems to
 // be adjacent elements of d and u, 16-byte aligned, and an even system
 // count), so a wave's row access covers 128 systems without gaps.
-// Non-Inc: d <- A^{-1} d; Inc: u <- u + A^{-1} d.
-template <typename T, int INC, int R, typename I, int NS>
+// Non-Inc: d <- A^{-1} d; Inc: u <- u + A^{-1} d.  CP: cache policy of the
+// streaming accesses (kStream / kCached).
+template <typename T, int INC, int R, typename I, int NS, int CP>
 __global__ void __launch_bounds__(cc_max_threads(R, NS))
     k_cc_sweep(const T *__restrict__ F, T *__restrict__ Dd, T *__restrict__ U, SysDesc D,
                int cps_log2, int xcd) {
@@ -206,11 +207,11 @@ __global__ void __launch_bounds__(cc_max_threads(R, NS))
     for (int r = 0; r < R; r++) {
       const bool ok = active && row0 + r < N;
       if (NS == 2) {
-        const V2 v = ok ? ld_nt(reinterpret_cast<const V2 *>(at(Dd, od))) : V2{};
+        const V2 v = ok ? ld_stream<CP>(reinterpret_cast<const V2 *>(at(Dd, od))) : V2{};
         d[0][r] = v[0];
         d[NS - 1][r] = v[1];
       } else {
-        d[0][r] = ok ? ld_nt(at(Dd, od)) : T(0);
+        d[0][r] = ok ? ld_stream<CP>(at(Dd, od)) : T(0);
       }
       od += s3;
     }
@@ -301,13 +302,13 @@ __global__ void __launch_bounds__(cc_max_threads(R, NS))
       if (row0 + r < N) {
         if (NS == 2) {
           V2 *pu = reinterpret_cast<V2 *>(at(U, ou));
-          V2 v = ld_nt(pu);
+          V2 v = ld_stream<CP>(pu);
           v[0] += y[0][r];
           v[1] += y[NS - 1][r];
-          st_nt(pu, v);
+          st_stream<CP>(pu, v);
         } else {
           T *pu = at(U, ou);
-          st_nt(pu, ld_nt(pu) + y[0][r]);
+          st_stream<CP>(pu, ld_stream<CP>(pu) + y[0][r]);
         }
       }
       ou += s4;
@@ -321,9 +322,9 @@ __global__ void __launch_bounds__(cc_max_threads(R, NS))
           V2 v;
           v[0] = y[0][r];
           v[1] = y[NS - 1][r];
-          st_nt(reinterpret_cast<V2 *>(at(Dd, od)), v);
+          st_stream<CP>(reinterpret_cast<V2 *>(at(Dd, od)), v);
         } else {
-          st_nt(at(Dd, od), y[0][r]);
+          st_stream<CP>(at(Dd, od), y[0][r]);
         }
       }
       od += s3;

@@ -17,10 +17,11 @@
 namespace tridhip {
 
 template <typename T, int INC, int ORDER, int R, int VEC, typename I,
-          typename Coef>
+          typename Coef, int CP>
 __global__ void __launch_bounds__(((R >= 16 && sizeof(T) == 8) || VEC == 4) ? 512 : 1024)
     k_part(Coef coef, T *__restrict__ Dd, T *__restrict__ U, SysDesc D,
            int cps_log2, int spb_log2, int xcd, T *__restrict__ Out) {
+  // CP: cache policy of the streaming accesses (kStream / kCached)
   // xcd: consecutive hardware blocks go round-robin to the 8 XCDs (each with
   // its own L2); this remap hands every XCD one contiguous range of systems
   // instead (a bijection on the first 8*floor(nblocks/8) blocks, identity on
@@ -30,8 +31,8 @@ __global__ void __launch_bounds__(((R >= 16 && sizeof(T) == 8) || VEC == 4) ? 51
     const unsigned nb8 = gridDim.x & ~7u;
     if (blockIdx.x < nb8) block = (long long)(blockIdx.x & 7u) * (nb8 >> 3) + (blockIdx.x >> 3);
   }
-  part_block<T, INC, ORDER, R, VEC, I>(coef, Dd, U, D, cps_log2, spb_log2,
-                                       block, Out);
+  part_block<T, INC, ORDER, R, VEC, I, Coef, 0, CP>(coef, Dd, U, D, cps_log2, spb_log2,
+                                                    block, Out);
 }
 
 // Segment steps of a split-line solve: D's systems are segments of longer
@@ -54,6 +55,7 @@ struct PartCfg {
   int vec = 0;
   int idx32 = 1;  // 32-bit element offsets (every array < 4 GiB)
   int xcd = 0;    // XCD-contiguous block order (see k_part)
+  int cp = 0;     // cache policy of the streaming accesses: kStream / kCached
 };
 
 inline bool aligned16(const void *p) {
@@ -84,13 +86,19 @@ hipError_t launch_part_t(const PartCfg &cfg, const Coef &coef, T *d, T *u,
   if (VEC == 4)
     lds = (size_t)(threads / kWave) * (kWave * R + 16 / sizeof(T) + (cps > kWave ? 12 : 0)) *
           sizeof(T);
+  // the cached policy exists for 32-bit offsets only: a solve whose arrays
+  // fit the last-level cache has arrays far below 4 GiB
+  if (cfg.idx32 && cfg.cp == kCached)
+    return launch_kernel(k_part<T, INC, ORDER, R, VEC, uint32_t, Coef, kCached>,
+                         dim3((unsigned)blocks), dim3(threads), (std::uint32_t)lds, st, coef, d,
+                         u, D, cfg.cps_log2, cfg.spb_log2, cfg.xcd, out);
   if (cfg.idx32)
-    return launch_kernel(k_part<T, INC, ORDER, R, VEC, uint32_t, Coef>, dim3((unsigned)blocks),
-                         dim3(threads), (std::uint32_t)lds, st, coef, d, u, D, cfg.cps_log2,
-                         cfg.spb_log2, cfg.xcd, out);
-  return launch_kernel(k_part<T, INC, ORDER, R, VEC, long long, Coef>, dim3((unsigned)blocks),
-                       dim3(threads), (std::uint32_t)lds, st, coef, d, u, D, cfg.cps_log2,
-                       cfg.spb_log2, cfg.xcd, out);
+    return launch_kernel(k_part<T, INC, ORDER, R, VEC, uint32_t, Coef, kStream>,
+                         dim3((unsigned)blocks), dim3(threads), (std::uint32_t)lds, st, coef, d,
+                         u, D, cfg.cps_log2, cfg.spb_log2, cfg.xcd, out);
+  return launch_kernel(k_part<T, INC, ORDER, R, VEC, long long, Coef, kStream>,
+                       dim3((unsigned)blocks), dim3(threads), (std::uint32_t)lds, st, coef, d, u,
+                       D, cfg.cps_log2, cfg.spb_log2, cfg.xcd, out);
 }
 
 // Launches the partitioned kernel for any configuration of the table.
@@ -267,11 +275,11 @@ inline std::string cfg_name(const PartCfg &c, size_t tsize) {
              tsize == 8 ? "double" : "float", c.order, c.R, c.R << c.cps_log2,
              1 << c.spb_log2, c.vec >= 2 ? ",staged" : (c.vec ? ",vec" : ""));
   } else {
-    snprintf(buf, sizeof buf, "part<%s,o%d,R%d,cps%d,spb%d%s%s>",
+    snprintf(buf, sizeof buf, "part<%s,o%d,R%d,cps%d,spb%d%s%s%s>",
              tsize == 8 ? "double" : "float", c.order, c.R, 1 << c.cps_log2,
              1 << c.spb_log2,
              c.vec == 4 ? ",gstaged" : (c.vec >= 2 ? ",staged" : (c.vec ? ",vec" : "")),
-             c.xcd ? ",xcd" : "");
+             c.xcd ? ",xcd" : "", c.cp == kCached ? ",cached" : "");
   }
   return buf;
 }

@@ -420,17 +420,30 @@ __device__ __forceinline__ void reduced_multi(const Row<T> &f, const Row<T> &l, 
   xf = f.d - f.a * xm - f.c * xl;
 }
 
-// Streaming (non-temporal) global accesses: every element of a, b, c, d
-// and u is touched exactly once per solve, so the loads and stores carry the
-// non-temporal hint.  On MI355X this lifts a read-4/write-1 stream from
-// about 68% to 79% of peak HBM bandwidth.
-template <typename T>
-__device__ __forceinline__ T ld_nt(const T *p) {
-  return __builtin_nontemporal_load(p);
+// Cache policy of the streaming global accesses (every element of a, b, c,
+// d and u is touched once per solve):
+//   kStream: non-temporal loads and stores, which do not allocate in L2 or
+//            in MI355X's 256 MB last-level cache (MALL).  Best when the
+//            arrays exceed that cache: a read-4/write-1 solve over 640 MiB
+//            runs at about 77 % of peak HBM bandwidth, against 68 % with
+//            ordinary accesses.
+//   kCached: ordinary accesses.  When the arrays of a solve, or of a chain
+//            of solves such as an ADI step (u and du), fit in the MALL, the
+//            next kernel (or call) reads what the previous one wrote from
+//            there: a copy -> in-place -> add chain over 2 x 128 MiB runs at
+//            93-97 % of 8 TB/s cached and at 64-77 % with non-temporal
+//            accesses.
+enum { kStream = 0, kCached = 1 };
+
+template <int CP = kStream, typename T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+  if constexpr (CP == kCached) return *p;
+  else return __builtin_nontemporal_load(p);
 }
-template <typename T>
-__device__ __forceinline__ void st_nt(T *p, T v) {
-  __builtin_nontemporal_store(v, p);
+template <int CP = kStream, typename T>
+__device__ __forceinline__ void st_stream(T *p, T v) {
+  if constexpr (CP == kCached) *p = v;
+  else __builtin_nontemporal_store(v, p);
 }
 
 template <typename T>
@@ -469,7 +482,7 @@ __device__ __forceinline__ T *at(T *base, I off) {
   }
 }
 
-template <typename T, int R>
+template <typename T, int R, int CP = kStream>
 __device__ __forceinline__ void load_chunk_vec(const T *__restrict__ p,
                                                T (&x)[R]) {
   using V = typename Vec16<T>::type;
@@ -478,13 +491,13 @@ __device__ __forceinline__ void load_chunk_vec(const T *__restrict__ p,
   const V *pv = reinterpret_cast<const V *>(p);
 #pragma unroll
   for (int v = 0; v < R / E; v++) {
-    V t = ld_nt(pv + v);
+    V t = ld_stream<CP>(pv + v);
 #pragma unroll
     for (int e = 0; e < E; e++) x[E * v + e] = t[e];
   }
 }
 
-template <typename T, int R>
+template <typename T, int R, int CP = kStream>
 __device__ __forceinline__ void store_chunk_vec(T *__restrict__ p,
                                                 const T (&x)[R]) {
   using V = typename Vec16<T>::type;
@@ -495,7 +508,7 @@ __device__ __forceinline__ void store_chunk_vec(T *__restrict__ p,
     V t;
 #pragma unroll
     for (int e = 0; e < E; e++) t[e] = x[E * v + e];
-    st_nt(pv + v, t);
+    st_stream<CP>(pv + v, t);
   }
 }
 
@@ -506,7 +519,7 @@ __device__ __forceinline__ void store_chunk_vec(T *__restrict__ p,
 // goes through a wave-private LDS buffer of 64*R elements.  LDS operations
 // of one wave execute in order, so no barrier is needed; wave_barrier only
 // keeps the compiler from reordering the LDS accesses.
-template <typename T, int R>
+template <typename T, int R, int CP = kStream>
 __device__ __forceinline__ void load_chunk_staged(const T *__restrict__ span,
                                                   T *__restrict__ buf,
                                                   int lane, T (&x)[R]) {
@@ -516,7 +529,7 @@ __device__ __forceinline__ void load_chunk_staged(const T *__restrict__ span,
   V *s = reinterpret_cast<V *>(buf);
   V t[R / E];
 #pragma unroll
-  for (int v = 0; v < R / E; v++) t[v] = ld_nt(g + v * kWave + lane);
+  for (int v = 0; v < R / E; v++) t[v] = ld_stream<CP>(g + v * kWave + lane);
 #pragma unroll
   for (int v = 0; v < R / E; v++) s[v * kWave + lane] = t[v];
   __builtin_amdgcn_wave_barrier();
@@ -532,7 +545,7 @@ __device__ __forceinline__ void load_chunk_staged(const T *__restrict__ span,
 
 // add: the span receives its own previous contents (or, with src, those of
 // src) plus x.
-template <typename T, int R>
+template <typename T, int R, int CP = kStream>
 __device__ __forceinline__ void store_chunk_staged(T *__restrict__ span,
                                                    T *__restrict__ buf,
                                                    int lane, const T (&x)[R],
@@ -555,8 +568,8 @@ __device__ __forceinline__ void store_chunk_staged(T *__restrict__ span,
 #pragma unroll
   for (int v = 0; v < R / E; v++) {
     V w = s[v * kWave + lane];
-    if (add) w += ld_nt(gs + v * kWave + lane);
-    st_nt(g + v * kWave + lane, w);
+    if (add) w += ld_stream<CP>(gs + v * kWave + lane);
+    st_stream<CP>(g + v * kWave + lane, w);
   }
   __builtin_amdgcn_wave_barrier();
 }
@@ -569,7 +582,7 @@ __device__ __forceinline__ void store_chunk_staged(T *__restrict__ span,
 // buf[sh + j], sh = start mod E.  A piece is the aligned 16-byte block that
 // holds a span element, so it never reaches past the page of that element.
 // Returns sh.
-template <typename T, int R, typename I>
+template <typename T, int R, typename I, int CP = kStream>
 __device__ __forceinline__ int gspan_load(const T *__restrict__ base, I start, int len,
                                           T *__restrict__ buf, int lane) {
   using V = typename Vec16<T>::type;
@@ -584,7 +597,7 @@ __device__ __forceinline__ int gspan_load(const T *__restrict__ base, I start, i
 #pragma unroll
   for (int v = 0; v <= R / E; v++) {
     const int pc = v * kWave + lane;
-    t[v] = ld_nt(g + (pc < np ? pc : np - 1));
+    t[v] = ld_stream<CP>(g + (pc < np ? pc : np - 1));
   }
 #pragma unroll
   for (int v = 0; v <= R / E; v++) {
@@ -610,7 +623,7 @@ __device__ __forceinline__ void gspan_take(const T *__restrict__ buf, int i0, in
 // boundary pieces one by one (the rest of those pieces belongs to other
 // waves).  add: the span receives its previous contents (or, with src,
 // those of src) plus buf.
-template <typename T, int R, typename I>
+template <typename T, int R, typename I, int CP = kStream>
 __device__ __forceinline__ void gspan_store(T *__restrict__ base, I start, int len,
                                             const T *__restrict__ buf, int lane, bool add,
                                             const T *src = nullptr) {
@@ -629,16 +642,16 @@ __device__ __forceinline__ void gspan_store(T *__restrict__ base, I start, int l
       const int lo = pc * E - sh;  // span index of the piece's first element
       V w = s[pc];
       if (lo >= 0 && lo + E <= len) {
-        if (add) w += ld_nt(gs + pc);
-        st_nt(g + pc, w);
+        if (add) w += ld_stream<CP>(gs + pc);
+        st_stream<CP>(g + pc, w);
       } else {
 #pragma unroll
         for (int e = 0; e < E; e++) {
           const int j = lo + e;
           if (j >= 0 && j < len) {
             T val = w[e];
-            if (add) val += ld_nt(at(srcb, start + (I)j));
-            st_nt(at(base, start + (I)j), val);
+            if (add) val += ld_stream<CP>(at(srcb, start + (I)j));
+            st_stream<CP>(at(base, start + (I)j), val);
           }
         }
       }
@@ -749,9 +762,9 @@ struct CoefTransposed {
     for (int r = 0; r < R; r++) {
       const int row = row0 + r;
       if (row < N) {
-        a[r] = row >= 1 ? ld_nt(at(C, off + (I)(row - 1) * s)) : T(0);
-        b[r] = ld_nt(at(B, off + (I)row * s));
-        c[r] = row + 1 < N ? ld_nt(at(A, off + (I)(row + 1) * s)) : T(0);
+        a[r] = row >= 1 ? ld_stream(at(C, off + (I)(row - 1) * s)) : T(0);
+        b[r] = ld_stream(at(B, off + (I)row * s));
+        c[r] = row + 1 < N ? ld_stream(at(A, off + (I)(row + 1) * s)) : T(0);
       } else {
         a[r] = T(0);
         b[r] = T(1);
@@ -826,7 +839,7 @@ struct CoefKappa {
 // (d_first -= a_first * last_{p-1}, d_last -= c_last * first_{p+1}) and the
 // segment is solved as an independent system.
 template <typename T, int INC, int ORDER, int R, int VEC, typename I,
-          typename Coef, int MODE = 0>
+          typename Coef, int MODE = 0, int CP = kStream>
 __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
                                            T *__restrict__ U, const SysDesc &D,
                                            int cps_log2, int spb_log2,
@@ -950,14 +963,14 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
     const int lane = tid & (kWave - 1);
     if (wave_active) {
       if constexpr (Coef::kArrays) {
-        int sh = gspan_load<T, R, I>(coef.A, wave_off[0], wave_len, stage, lane);
+        int sh = gspan_load<T, R, I, CP>(coef.A, wave_off[0], wave_len, stage, lane);
         gspan_take<T, R>(stage, sh + g_idx, g_rows, a, T(0));
-        sh = gspan_load<T, R, I>(coef.B, wave_off[1], wave_len, stage, lane);
+        sh = gspan_load<T, R, I, CP>(coef.B, wave_off[1], wave_len, stage, lane);
         gspan_take<T, R>(stage, sh + g_idx, g_rows, b, T(1));
-        sh = gspan_load<T, R, I>(coef.C, wave_off[2], wave_len, stage, lane);
+        sh = gspan_load<T, R, I, CP>(coef.C, wave_off[2], wave_len, stage, lane);
         gspan_take<T, R>(stage, sh + g_idx, g_rows, c, T(0));
       }
-      const int sh = gspan_load<T, R, I>(Dd, wave_off[3], wave_len, stage, lane);
+      const int sh = gspan_load<T, R, I, CP>(Dd, wave_off[3], wave_len, stage, lane);
       gspan_take<T, R>(stage, sh + g_idx, g_rows, d, T(0));
     } else {
 #pragma unroll
@@ -977,11 +990,11 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
     const int lane = tid & (kWave - 1);
     if (wave_active) {
       if constexpr (Coef::kArrays) {
-        load_chunk_staged<T, R>(at(coef.A, wave_off[0]), stage, lane, a);
-        load_chunk_staged<T, R>(at(coef.B, wave_off[1]), stage, lane, b);
-        load_chunk_staged<T, R>(at(coef.C, wave_off[2]), stage, lane, c);
+        load_chunk_staged<T, R, CP>(at(coef.A, wave_off[0]), stage, lane, a);
+        load_chunk_staged<T, R, CP>(at(coef.B, wave_off[1]), stage, lane, b);
+        load_chunk_staged<T, R, CP>(at(coef.C, wave_off[2]), stage, lane, c);
       }
-      load_chunk_staged<T, R>(at(Dd, wave_off[3]), stage, lane, d);
+      load_chunk_staged<T, R, CP>(at(Dd, wave_off[3]), stage, lane, d);
     } else {
 #pragma unroll
       for (int r = 0; r < R; r++) {
@@ -997,20 +1010,20 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
     // (staged layouts never get here: VEC is 0 or 1)
     if constexpr (VEC == 1) {
       if constexpr (Coef::kArrays) {
-        load_chunk_vec<T, R>(at(coef.A, oa), a);
-        load_chunk_vec<T, R>(at(coef.B, ob), b);
-        load_chunk_vec<T, R>(at(coef.C, oc), c);
+        load_chunk_vec<T, R, CP>(at(coef.A, oa), a);
+        load_chunk_vec<T, R, CP>(at(coef.B, ob), b);
+        load_chunk_vec<T, R, CP>(at(coef.C, oc), c);
       }
-      load_chunk_vec<T, R>(at(Dd, od), d);
+      load_chunk_vec<T, R, CP>(at(Dd, od), d);
     } else {
 #pragma unroll
       for (int r = 0; r < R; r++) {
         if constexpr (Coef::kArrays) {
-          a[r] = ld_nt(at(coef.A, oa));
-          b[r] = ld_nt(at(coef.B, ob));
-          c[r] = ld_nt(at(coef.C, oc));
+          a[r] = ld_stream<CP>(at(coef.A, oa));
+          b[r] = ld_stream<CP>(at(coef.B, ob));
+          c[r] = ld_stream<CP>(at(coef.C, oc));
         }
-        d[r] = ld_nt(at(Dd, od));
+        d[r] = ld_stream<CP>(at(Dd, od));
         oa += s0;
         ob += s1;
         oc += s2;
@@ -1023,11 +1036,11 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
       const int row = row0 + r;
       if (active && row < N) {
         if constexpr (Coef::kArrays) {
-          a[r] = row == 0 ? T(0) : ld_nt(at(coef.A, off[0] + (I)row * s0));
-          b[r] = ld_nt(at(coef.B, off[1] + (I)row * s1));
-          c[r] = row == N - 1 ? T(0) : ld_nt(at(coef.C, off[2] + (I)row * s2));
+          a[r] = row == 0 ? T(0) : ld_stream<CP>(at(coef.A, off[0] + (I)row * s0));
+          b[r] = ld_stream<CP>(at(coef.B, off[1] + (I)row * s1));
+          c[r] = row == N - 1 ? T(0) : ld_stream<CP>(at(coef.C, off[2] + (I)row * s2));
         }
-        d[r] = ld_nt(at(Dd, off[3] + (I)row * s3));
+        d[r] = ld_stream<CP>(at(Dd, off[3] + (I)row * s3));
       } else {
         a[r] = T(0);
         b[r] = T(1);
@@ -1168,44 +1181,44 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
         if (r < g_rows) stage[sh + g_idx + r] = y[r];
       __builtin_amdgcn_wave_barrier();
       if (INC)
-        gspan_store<T, R, I>(Udst, wave_off[4], wave_len, stage, lane, true,
+        gspan_store<T, R, I, CP>(Udst, wave_off[4], wave_len, stage, lane, true,
                              static_cast<const T *>(U));
       else
-        gspan_store<T, R, I>(Dst, wave_off[3], wave_len, stage, lane, false);
+        gspan_store<T, R, I, CP>(Dst, wave_off[3], wave_len, stage, lane, false);
     }
   } else if (STAGED) {
     const int lane = tid & (kWave - 1);
     if (wave_active) {
       if (INC)
-        store_chunk_staged<T, R>(at(Udst, wave_off[4]), stage, lane, y, true,
+        store_chunk_staged<T, R, CP>(at(Udst, wave_off[4]), stage, lane, y, true,
                                  at(static_cast<const T *>(U), wave_off[4]));
       else
-        store_chunk_staged<T, R>(at(Dst, wave_off[3]), stage, lane, y, false);
+        store_chunk_staged<T, R, CP>(at(Dst, wave_off[3]), stage, lane, y, false);
     }
   } else if (full) {
     if (INC) {
       I ou = off[4] + (I)row0 * s4;
       if constexpr (VEC == 1) {
         T u[R];
-        load_chunk_vec<T, R>(at(U, ou), u);
+        load_chunk_vec<T, R, CP>(at(U, ou), u);
 #pragma unroll
         for (int r = 0; r < R; r++) u[r] += y[r];
-        store_chunk_vec<T, R>(at(Udst, ou), u);
+        store_chunk_vec<T, R, CP>(at(Udst, ou), u);
       } else {
 #pragma unroll
         for (int r = 0; r < R; r++) {
-          st_nt(at(Udst, ou), ld_nt(at(U, ou)) + y[r]);
+          st_stream<CP>(at(Udst, ou), ld_stream<CP>(at(U, ou)) + y[r]);
           ou += s4;
         }
       }
     } else {
       I od = off[3] + (I)row0 * s3;
       if constexpr (VEC == 1) {
-        store_chunk_vec<T, R>(at(Dst, od), y);
+        store_chunk_vec<T, R, CP>(at(Dst, od), y);
       } else {
 #pragma unroll
         for (int r = 0; r < R; r++) {
-          st_nt(at(Dst, od), y[r]);
+          st_stream<CP>(at(Dst, od), y[r]);
           od += s3;
         }
       }
@@ -1217,9 +1230,9 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
       if (row < N) {
         if (INC) {
           const I ou = off[4] + (I)row * s4;
-          st_nt(at(Udst, ou), ld_nt(at(U, ou)) + y[r]);
+          st_stream<CP>(at(Udst, ou), ld_stream<CP>(at(U, ou)) + y[r]);
         } else {
-          st_nt(at(Dst, off[3] + (I)row * s3), y[r]);
+          st_stream<CP>(at(Dst, off[3] + (I)row * s3), y[r]);
         }
       }
     }

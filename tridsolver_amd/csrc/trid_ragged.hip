@@ -104,7 +104,7 @@ __device__ __forceinline__ int rag_load(const T *__restrict__ p, int len, T *__r
 #pragma unroll
   for (int j = 0; j < NR; j++) {
     const int pc = j * G + kg;
-    t[j] = ld_nt(g + (pc < np ? pc : last));
+    t[j] = ld_stream(g + (pc < np ? pc : last));
   }
   V *s = reinterpret_cast<V *>(reg);
 #pragma unroll
@@ -146,16 +146,16 @@ __device__ __forceinline__ void rag_store(T *__restrict__ p, int len, const T *_
       const int lo = pc * E - sh;
       V w = s[pc];
       if (lo >= 0 && lo + E <= len) {
-        if (add) w += ld_nt(g + pc);
-        st_nt(g + pc, w);
+        if (add) w += ld_stream(g + pc);
+        st_stream(g + pc, w);
       } else {
 #pragma unroll
         for (int e = 0; e < E; e++) {
           const int i = lo + e;
           if (i >= 0 && i < len) {
             T v = w[e];
-            if (add) v += ld_nt(p + i);
-            st_nt(p + i, v);
+            if (add) v += ld_stream(p + i);
+            st_stream(p + i, v);
           }
         }
       }

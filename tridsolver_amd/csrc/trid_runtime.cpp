@@ -19,6 +19,7 @@ namespace {
 std::atomic<hipStream_t> g_stream{nullptr};
 std::atomic<int> g_autotune{-1};
 std::atomic<int> g_wide{0};
+std::atomic<int> g_cache_policy{-1};
 std::mutex g_mu;
 ForcedConfig g_forced;
 thread_local std::string t_error;
@@ -125,6 +126,14 @@ int autotune_mode() {
 
 bool wide_index() { return g_wide.load() != 0; }
 
+int forced_cache_policy() { return g_cache_policy.load(); }
+
+int cache_policy(size_t bytes) {
+  const int f = g_cache_policy.load();
+  if (f >= 0) return f;
+  return bytes <= kMallFitBytes ? 1 : 0;
+}
+
 ForcedConfig forced_config() {
   std::lock_guard<std::mutex> lk(g_mu);
   return g_forced;
@@ -193,5 +202,12 @@ EXTERN_C tridStatus_t tridForceConfig(int kind, int rows, int cps, int spb) {
 EXTERN_C tridStatus_t tridForceWideIndex(int on) {
   if (on != 0 && on != 1) return fail(TRID_STATUS_INVALID_VALUE, "tridForceWideIndex: on must be 0 or 1");
   g_wide.store(on);
+  return TRID_STATUS_SUCCESS;
+}
+
+EXTERN_C tridStatus_t tridForceCachePolicy(int policy) {
+  if (policy < -1 || policy > 1)
+    return fail(TRID_STATUS_INVALID_VALUE, "tridForceCachePolicy: policy must be -1, 0 or 1");
+  g_cache_policy.store(policy);
   return TRID_STATUS_SUCCESS;
 }

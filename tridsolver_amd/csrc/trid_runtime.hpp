@@ -62,6 +62,17 @@ inline hipError_t launch_kernel(F kernel, dim3 grid, dim3 block, std::uint32_t l
   return hipGetLastError();
 }
 
+// Cache policy of the streaming accesses (0 non-temporal, 1 ordinary; the
+// kernels' kStream / kCached) for a solve, or a chain of solves such as an
+// ADI step, whose arrays span `bytes`: ordinary accesses when they fit in
+// about 1.5x MI355X's 256 MB last-level cache (kMallFitBytes), so that each
+// kernel finds much of what the previous one (or call) wrote there;
+// non-temporal past it.  tridForceCachePolicy overrides the rule.
+constexpr size_t kMallFitBytes = 384ull << 20;
+int cache_policy(size_t bytes);
+// The policy tridForceCachePolicy fixed, -1 when none is.
+int forced_cache_policy();
+
 // True while tridForceWideIndex(1) is in effect: the kernels then use 64-bit
 // element offsets and 64-bit system-index arithmetic whatever the sizes.
 bool wide_index();

@@ -310,17 +310,24 @@ tridStatus_t run_cfg(const PartCfg &cfg, const T *a, const T *b, const T *c,
   return check_hip(e, "kernel launch");
 }
 
+// cp: the cache policy the candidates run with; unless a policy is forced
+// (tridForceCachePolicy), the winner is then timed against its twin under
+// the other policy, which it keeps unless the twin is more than 2 % faster.
 template <typename T, int INC>
 PartCfg autotune(int order, int vec, bool gvec, int idx32, bool dense, const T *a,
                  const T *b, const T *c, T *d, T *u, const SysDesc &D,
-                 long long d_extent, hipStream_t st) {
+                 long long d_extent, int cp, hipStream_t st) {
   std::vector<PartCfg> cands =
       candidates(order, D.N, vec, sizeof(T), dense, D.nsys, gvec);
-  for (auto &cf : cands) cf.idx32 = idx32;
+  for (auto &cf : cands) {
+    cf.idx32 = idx32;
+    cf.cp = idx32 ? cp : kStream;
+  }
   PartCfg best = heuristic(order, D.N, vec, sizeof(T));
   if (stage_ok(best, dense, D.nsys, D.N)) best.vec = staged_vec(best);
   else if (gstage_ok(best, gvec, dense)) best.vec = 4;
   best.idx32 = idx32;
+  best.cp = idx32 ? cp : kStream;
   if (cands.size() < 2) return best;
   // the candidates run on the caller's own arrays (memory-channel effects of
   // large power-of-two strides depend on the actual addresses), repeatedly
@@ -343,6 +350,18 @@ PartCfg autotune(int order, int vec, bool gvec, int idx32, bool dense, const T *
     for (size_t c = 0; c < cands.size(); c++)
       fprintf(stderr, "[tridhip tune] N=%d nsys=%lld order %d %s: %.4f ms\n", D.N, D.nsys,
               order, cfg_name(cands[c], sizeof(T)).c_str(), tm[c]);
+  if (i >= 0 && idx32 && forced_cache_policy() < 0) {
+    // the same layout under the other cache policy
+    PartCfg twin = best;
+    twin.cp = best.cp == kCached ? kStream : kCached;
+    cands.assign({best, twin});
+    const int j = pick_fastest(2, prep, run, st, &tm);
+    if (j >= 0) best = cands[j];
+    if (log && log[0] == '1')
+      fprintf(stderr, "[tridhip tune] N=%d nsys=%lld order %d cache policy: %s %.4f ms, %s %.4f ms\n",
+              D.N, D.nsys, order, cfg_name(cands[0], sizeof(T)).c_str(), tm[0],
+              cfg_name(cands[1], sizeof(T)).c_str(), tm[1]);
+  }
   (void)hipMemcpyAsync(dst, saved, bytes, hipMemcpyDeviceToDevice, st);
   return best;
 }
@@ -424,11 +443,14 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
   D.small_index = D.nsys < (1LL << 31) && !wide_index();
   // 32-bit offsets when every array (its allocated extent) fits in 4 GiB
   int idx32 = !wide_index();
+  size_t footprint = 0;  // bytes spanned by the arrays the solve touches
   for (int t = 0; t < 5; t++) {
     if (t == 4 && !INC) break;
-    if ((unsigned long long)extent(ndim, dims, pads[t]) * sizeof(T) >= (1ULL << 32))
-      idx32 = 0;
+    const unsigned long long bytes = (unsigned long long)extent(ndim, dims, pads[t]) * sizeof(T);
+    if (bytes >= (1ULL << 32)) idx32 = 0;
+    footprint += bytes;
   }
+  const int cp_rule = cache_policy(footprint);
 
   const int order = solvedim == 0 ? 0 : 1;
   // vector loads along the system need unit stride and 16-byte aligned chunks
@@ -457,6 +479,7 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
   }
   hipStream_t st = current_stream();
   PartCfg cfg;
+  bool tuned = false;  // cfg (and its cache policy) from the autotuner
   ForcedConfig fc = forced_config();
   if (fc.kind == 2) {
     cfg.kind = 2;
@@ -500,6 +523,7 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
     // (a call with fixed scratch never tunes: tuning allocates)
     bool have = false;
     const bool tune = autotune_mode() == 1 && !fixed;
+    tuned = tune;
     auto key = std::make_tuple((int)sizeof(T), INC, order, D.N, D.nsys,
                                vec * 4 + (int)gvec * 2 + (int)dense, D.sstr[3],
                                D.nother > 0 ? D.ostr[3][0] : 0LL);
@@ -513,12 +537,16 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
     }
     if (!have && tune && !capturing(st)) {
       long long ext = extent(ndim, dims, INC ? pads[4] : pads[3]);
-      cfg = autotune<T, INC>(order, vec, gvec, idx32, dense, a, b, c, d, u, D, ext, st);
+      cfg = autotune<T, INC>(order, vec, gvec, idx32, dense, a, b, c, d, u, D, ext, cp_rule,
+                             st);
       std::lock_guard<std::mutex> lk(g_tune_mu);
       g_tuned[key] = cfg;
       have = true;
     }
-    if (!have) cfg = default_cfg<T>(order, D, vec, gvec, dense);
+    if (!have) {
+      cfg = default_cfg<T>(order, D, vec, gvec, dense);
+      tuned = false;
+    }
   }
   // systems longer than one work-group holds: split-line solve.  Strided
   // systems (ORDER 1) of 2048 rows and more take it too: a work-group that
@@ -531,6 +559,11 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
   if (cfg.kind == 4 && D.nsys > (1LL << 30))
     return fail(TRID_STATUS_INVALID_VALUE, "split-line solve: too many systems");
   cfg.idx32 = idx32;
+  // cache policy: the tuned one, else the footprint rule; a forced policy
+  // wins; split-line, fallback and 64-bit-offset solves always stream
+  if (!tuned) cfg.cp = cp_rule;
+  if (forced_cache_policy() >= 0) cfg.cp = forced_cache_policy();
+  if (!idx32 || cfg.kind != 1) cfg.cp = kStream;
   set_last_config(cfg_name(cfg, sizeof(T)));
   // launch events (tridSetLaunchEvents) time single-kernel solves only
   if (cfg.kind != 1) (void)take_launch_events();

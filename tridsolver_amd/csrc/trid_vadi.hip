@@ -95,7 +95,8 @@ __global__ void __launch_bounds__(256)
 // of the lines per wave and 16-byte aligned arrays).
 //   MODE 0: out = A_x^{-1} D(kappa) in   (in = u_k, out = p)
 //   MODE 1: out = A_x^{-T} in
-template <int R, int MODE>
+// CP: cache policy of the output stores (kStream / kCached).
+template <int R, int MODE, int CP>
 __global__ void __launch_bounds__(256)
     k_vadi_x(const double *__restrict__ K, const double *__restrict__ in,
              double *__restrict__ out, int nx, int ny, int nz, double lam, int cps_log2) {
@@ -230,7 +231,7 @@ __global__ void __launch_bounds__(256)
   reduced_wave<double>(f, l, kc_idx, cps, lane, xf, xl);
   double y[R];
   chunk_expand<double, R>(a, c, d, xf, xl, y);
-  if (wave_active) store_chunk_staged<double, R>(out + wo, stage, lane, y, false);
+  if (wave_active) store_chunk_staged<double, R, CP>(out + wo, stage, lane, y, false);
 }
 
 // Adjoint of one step at every point p (gather form, no atomics).  For each
@@ -424,6 +425,14 @@ SysDesc line_desc(const Grid &g, int axis) {
   return D;
 }
 
+// Cache policy of the streaming accesses, from the working set of a forward
+// and adjoint step: about ten grid arrays (u, kappa, the sweep outputs,
+// their adjoints, the gradients).
+int vadi_policy(const Grid &g) {
+  if ((unsigned long long)g.n * 8 >= (1ULL << 32) || wide_index()) return kStream;
+  return cache_policy(10 * (size_t)g.n * sizeof(double));
+}
+
 // Generic kappa sweep along `axis` (partitioned Thomas, scalar loads):
 // non-inc: out (default d) = A^{-1} d or A^{-T} d; inc: out (default u) =
 // u + A^{-1} d.
@@ -435,6 +444,7 @@ tridStatus_t vadi_sweep(const double *K, double *d, double *u, double *out, cons
   if (cfg.kind != 1) return fail(TRID_STATUS_INVALID_VALUE, "vadi: grid too long along an axis");
   cfg.vec = 0;
   cfg.idx32 = (unsigned long long)g.n * 8 < (1ULL << 32) && !wide_index();
+  cfg.cp = cfg.idx32 ? vadi_policy(g) : kStream;
   const CoefKappa<double> cf{K, lam, transpose};
   hipError_t e = inc ? launch_part<double, 1>(cfg, cf, d, u, D, st, out)
                      : launch_part<double, 0>(cfg, cf, d, u, D, st, out);
@@ -469,15 +479,20 @@ tridStatus_t launch_vadi_x(int R, const double *K, const double *in, double *out
   const long long nlines = (long long)g.ny * g.nz;
   const unsigned blocks = (unsigned)((nlines + lines_per_block - 1) / lines_per_block);
   const size_t lds = 256 * (size_t)R * sizeof(double);
-  if (R == 8)
-    hipLaunchKernelGGL((k_vadi_x<8, MODE>), dim3(blocks), dim3(256), lds, st, K, in, out,
-                       g.nx, g.ny, g.nz, lam, cps_log2);
-  else if (R == 4)
-    hipLaunchKernelGGL((k_vadi_x<4, MODE>), dim3(blocks), dim3(256), lds, st, K, in, out,
-                       g.nx, g.ny, g.nz, lam, cps_log2);
-  else
-    hipLaunchKernelGGL((k_vadi_x<16, MODE>), dim3(blocks), dim3(256), lds, st, K, in, out,
-                       g.nx, g.ny, g.nz, lam, cps_log2);
+#define TRID_VX(RR, CC)                                                                  \
+  hipLaunchKernelGGL((k_vadi_x<RR, MODE, CC>), dim3(blocks), dim3(256), lds, st, K, in, out, \
+                     g.nx, g.ny, g.nz, lam, cps_log2)
+#define TRID_VX_R(CC)              \
+  if (R == 8) TRID_VX(8, CC);      \
+  else if (R == 4) TRID_VX(4, CC); \
+  else TRID_VX(16, CC);
+  if (vadi_policy(g) == kCached) {
+    TRID_VX_R(kCached)
+  } else {
+    TRID_VX_R(kStream)
+  }
+#undef TRID_VX_R
+#undef TRID_VX
   return check_hip(hipGetLastError(), "vadi x launch");
 }
 
