@@ -90,7 +90,8 @@ def kernel_of_config(cfg: str, dtype: str, inc: int, coef: str):
     library (tridGetLastConfig, an ADI plan's stage), as rocprofv3 names it."""
     import re
     T = "double" if dtype == "f64" else "float"
-    # the last template argument is the cache policy (0 non-temporal, 1 ordinary)
+    # the last template argument is the cache policy (0 non-temporal, 1 ordinary,
+    # 2 non-temporal coefficients with ordinary d and u)
     m = re.search(r"cc<R(\d+),cps\d+((?:,\w+)*)>", cfg)
     if m:
         fl = m.group(2).split(",")
@@ -102,8 +103,9 @@ def kernel_of_config(cfg: str, dtype: str, inc: int, coef: str):
         staged = "staged" in fl
         vec = 4 if "gstaged" in fl else (3 if staged and int(m.group(3)) > 64 else
                                          (2 if staged else (1 if "vec" in fl else 0)))
+        cp = 1 if "cached" in fl else (2 if "mixed" in fl else 0)
         return (f"k_part<{T}, {inc}, {m.group(1)}, {m.group(2)}, {vec}, unsigned int, "
-                f"{coef}<{T}>, {int('cached' in fl)}>")
+                f"{coef}<{T}>, {cp}>")
     m = re.search(r"rhs\+x\(fused,R(\d+)((?:,\w+)*)\)", cfg)
     if m:
         return f"k_adi_rhs_x<{T}, {m.group(1)}, {int('cached' in m.group(2).split(','))}>"

@@ -71,13 +71,17 @@ EXTERN_C tridStatus_t tridForceWideIndex(int on);
  * right-hand-side and solution arrays): policy 0 = non-temporal (they do not
  * allocate in L2 or the 256 MB last-level cache; best when the arrays exceed
  * that cache), 1 = ordinary (arrays that fit stay in the last-level cache,
- * so the next kernel or call reads them from there), -1 (the default) =
- * automatic: ordinary when the arrays of the call -- of an ADI plan: u and
- * du -- span at most 384 MiB, else non-temporal.  Applies to later strided
- * solves and to ADI plans created afterwards; both policies give bitwise
- * equal results.  Solves with 64-bit element offsets are always
- * non-temporal.  No counterpart in the reference (its kernels have no cache
- * hints). */
+ * so the next kernel or call reads them from there), 2 = non-temporal
+ * coefficient arrays a, b, c and ordinary d and u (the same as 1 where the
+ * coefficients are not arrays: ADI plans, the variable-diffusivity ADI),
+ * -1 (the default) = automatic: ordinary when the arrays of the call -- of
+ * an ADI plan: u and du -- span at most 384 MiB, else 2 when d and u do,
+ * else non-temporal; with autotuning on, strided solves also time the other
+ * policies.  Applies to
+ * later strided solves and to ADI plans created afterwards; all policies
+ * give bitwise equal results.  Solves with 64-bit element offsets are
+ * always non-temporal.  No counterpart in the reference (its kernels have no
+ * cache hints). */
 EXTERN_C tridStatus_t tridForceCachePolicy(int policy);
 
 /* Diagnostic: y[i] = the solvers' reciprocal of x[i] for n device values

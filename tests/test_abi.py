@@ -68,10 +68,10 @@ def test_library_loads_without_gpu():
 
 def test_cache_policy_switch_validates():
     import tridsolver_amd as ta
-    for bad in (2, -2):
+    for bad in (3, -2):
         with pytest.raises(ta.TridError):
             ta.force_cache_policy(bad)
-    for ok in (0, 1, -1):
+    for ok in (0, 1, 2, -1):
         ta.force_cache_policy(ok)
 
 

@@ -1,9 +1,10 @@
 """GPU parity of the two cache policies of the streaming kernels.
 
 The partitioned kernels, the factor-reuse sweeps and the ADI right-hand-side
-kernels exist with non-temporal loads and stores (policy 0) and with ordinary
+kernels exist with non-temporal loads and stores (policy 0), with ordinary
 ones (policy 1, which the automatic rule picks for the small arrays of the
-other GPU tests).  The policy changes no arithmetic, so both must give
+other GPU tests) and, for coefficient arrays, with non-temporal a, b, c and
+ordinary d, u (policy 2).  The policy changes no arithmetic, so both must give
 bitwise equal results, each within the usual tolerance of the oracle (1e-12
 FP64, 1e-5 FP32 normwise); the configuration text names the policy.
 """
@@ -51,10 +52,12 @@ def test_strided_policies_bitwise_equal(policy_reset, cfg, sd, dtype, inc):
     ta.force_config(*cfg)
     g0, c0 = solve_under(0, a, b, c, d, u, dims, sd, pads, inc)
     g1, c1 = solve_under(1, a, b, c, d, u, dims, sd, pads, inc)
-    assert ",cached" not in c0
+    g2, c2 = solve_under(2, a, b, c, d, u, dims, sd, pads, inc)
+    assert ",cached" not in c0 and ",mixed" not in c0
     if c1.startswith("part<"):
-        assert ",cached" in c1, c1
+        assert ",cached" in c1 and ",mixed" in c2, (c1, c2)
     np.testing.assert_array_equal(g0, g1)
+    np.testing.assert_array_equal(g0, g2)
     exp = oracle_solve(a, b, c, d, u, dims, sd, pads, inc)
     check_close(exp, g1, dtype, f"dims={dims} sd={sd} inc={inc} [{c1}]")
 
@@ -69,27 +72,37 @@ def test_staged_policies_bitwise_equal(policy_reset, dims, sd, dtype):
         u = make_u(dims, dims, dtype) if inc else None
         g0, _ = solve_under(0, a, b, c, d, u, dims, sd, dims, inc)
         g1, c1 = solve_under(1, a, b, c, d, u, dims, sd, dims, inc)
+        g2, _ = solve_under(2, a, b, c, d, u, dims, sd, dims, inc)
         np.testing.assert_array_equal(g0, g1)
+        np.testing.assert_array_equal(g0, g2)
         exp = oracle_solve(a, b, c, d, u, dims, sd, dims, inc)
         check_close(exp, g1, dtype, f"dims={dims} sd={sd} inc={inc} [{c1}]")
 
 
 def test_automatic_policy_follows_footprint(policy_reset):
-    """ordinary accesses for a small batch, non-temporal past 384 MiB"""
+    """ordinary accesses while all arrays span at most 384 MiB, then
+    non-temporal coefficients with ordinary d while d does, then
+    non-temporal"""
     import torch
     ta.force_cache_policy(-1)
     dims = [64, 64, 64]
     a, b, c, d = dd_system(dims, 1, np.float64, seed=1, pads=dims)
     gpu_solve(a, b, c, d, None, dims, 1, dims, False)
     assert ",cached" in ta.last_config()
-    # 4 arrays x 128 MiB: past the rule's 384 MiB
-    n = 256
-    t = [torch.full((n, n, n), v, dtype=torch.float64, device="cuda") for v in (-1.0, 4.0, -1.0, 1.0)]
-    st = ta.api.tridDmtsvStridedBatch(*t, None, 3, 1, [n] * 3, [n] * 3, [0] * 3, 1)
-    assert st == 0, ta.last_error()
-    assert ",cached" not in ta.last_config()
-    del t
-    torch.cuda.empty_cache()
+    for shape, tag in (((256, 256, 256), ",mixed"), ((512, 512, 256), "")):
+        # 4 arrays of 128 MiB (d fits) and of 512 MiB (d does not)
+        t = [torch.full(shape, v, dtype=torch.float64, device="cuda") for v in (-1.0, 4.0, -1.0, 1.0)]
+        dims = list(reversed(shape))
+        st = ta.api.tridDmtsvStridedBatch(*t, None, 3, 1, dims, dims, [0] * 3, 1)
+        assert st == 0, ta.last_error()
+        cfg = ta.last_config()
+        assert ",cached" not in cfg and (tag in cfg if tag else ",mixed" not in cfg), cfg
+        # the solution of tridiag(-1, 4, -1) x = 1 along y: symmetric, in (1/4, 1/2]
+        x = t[3][0, :, 0].cpu().numpy()
+        assert np.all(x > 0.25) and np.all(x <= 0.5 + 1e-14)
+        np.testing.assert_allclose(x, x[::-1], rtol=0, atol=1e-14)
+        del t
+        torch.cuda.empty_cache()
 
 
 @pytest.mark.parametrize("shape", [(64, 40, 24), (256, 24, 30), (33, 17, 9)])

@@ -86,8 +86,10 @@ def force_wide_index(on: bool) -> None:
 def force_cache_policy(policy: int) -> None:
     """Cache policy of the solvers' streaming loads and stores: 0 non-temporal,
     1 ordinary (arrays that fit the 256 MB last-level cache stay there for
-    the next kernel or call), -1 automatic (ordinary up to 384 MiB of arrays
-    per call or ADI step).  Both give bitwise equal results."""
+    the next kernel or call), 2 non-temporal coefficient arrays with ordinary
+    d and u, -1 automatic (ordinary up to 384 MiB of arrays per call or ADI
+    step, else 2 while d and u fit in 384 MiB, else 0).  All give bitwise
+    equal results."""
     _check(lib().tridForceCachePolicy(int(policy)), "tridForceCachePolicy")
 
 

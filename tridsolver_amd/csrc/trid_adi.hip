@@ -756,6 +756,7 @@ EXTERN_C tridStatus_t tridAdiPlanCreate(tridAdiPlan_t *plan, int nx, int ny,
   const int idx32 = (unsigned long long)nx * ny * nz * ts < (1ULL << 32) && !wide_index();
   // cache policy from the step's working set, u and du
   p->cp = idx32 ? cache_policy(2 * (size_t)nx * ny * nz * ts) : kStream;
+  if (p->cp == kMixed) p->cp = kCached;  // no coefficient arrays: d and u cached
   const int n[3] = {nx, ny, nz};
   for (int dir = 0; dir < 3; dir++) {
     p->desc[dir] = grid_desc(nx, ny, nz, dir);

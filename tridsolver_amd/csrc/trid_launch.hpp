@@ -55,7 +55,7 @@ struct PartCfg {
   int vec = 0;
   int idx32 = 1;  // 32-bit element offsets (every array < 4 GiB)
   int xcd = 0;    // XCD-contiguous block order (see k_part)
-  int cp = 0;     // cache policy of the streaming accesses: kStream / kCached
+  int cp = 0;     // cache policy of the streaming accesses: kStream / kCached / kMixed
 };
 
 inline bool aligned16(const void *p) {
@@ -92,6 +92,12 @@ hipError_t launch_part_t(const PartCfg &cfg, const Coef &coef, T *d, T *u,
     return launch_kernel(k_part<T, INC, ORDER, R, VEC, uint32_t, Coef, kCached>,
                          dim3((unsigned)blocks), dim3(threads), (std::uint32_t)lds, st, coef, d,
                          u, D, cfg.cps_log2, cfg.spb_log2, cfg.xcd, out);
+  if constexpr (Coef::kArrays) {
+    if (cfg.idx32 && cfg.cp == kMixed)
+      return launch_kernel(k_part<T, INC, ORDER, R, VEC, uint32_t, Coef, kMixed>,
+                           dim3((unsigned)blocks), dim3(threads), (std::uint32_t)lds, st, coef,
+                           d, u, D, cfg.cps_log2, cfg.spb_log2, cfg.xcd, out);
+  }
   if (cfg.idx32)
     return launch_kernel(k_part<T, INC, ORDER, R, VEC, uint32_t, Coef, kStream>,
                          dim3((unsigned)blocks), dim3(threads), (std::uint32_t)lds, st, coef, d,
@@ -279,7 +285,8 @@ inline std::string cfg_name(const PartCfg &c, size_t tsize) {
              tsize == 8 ? "double" : "float", c.order, c.R, 1 << c.cps_log2,
              1 << c.spb_log2,
              c.vec == 4 ? ",gstaged" : (c.vec >= 2 ? ",staged" : (c.vec ? ",vec" : "")),
-             c.xcd ? ",xcd" : "", c.cp == kCached ? ",cached" : "");
+             c.xcd ? ",xcd" : "",
+             c.cp == kCached ? ",cached" : (c.cp == kMixed ? ",mixed" : ""));
   }
   return buf;
 }

@@ -433,7 +433,11 @@ __device__ __forceinline__ void reduced_multi(const Row<T> &f, const Row<T> &l, 
 //            there: a copy -> in-place -> add chain over 2 x 128 MiB runs at
 //            93-97 % of 8 TB/s cached and at 64-77 % with non-temporal
 //            accesses.
-enum { kStream = 0, kCached = 1 };
+//   kMixed:  (coefficient arrays only) non-temporal a, b, c and ordinary d
+//            and u: the read-only coefficients do not evict the right-hand
+//            side, which a caller's previous kernel wrote and its next one
+//            reads, from the last-level cache.
+enum { kStream = 0, kCached = 1, kMixed = 2 };
 
 template <int CP = kStream, typename T>
 __device__ __forceinline__ T ld_stream(const T *p) {
@@ -848,6 +852,10 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
                                            SegOut<T> so = SegOut<T>()) {
   static_assert(!Coef::kHook || VEC == 0, "hooked coefficients load scalars");
   static_assert(MODE == 0 || Coef::kArrays, "segment reduction reads coefficient arrays");
+  static_assert(CP != kMixed || Coef::kArrays, "the mixed policy is for coefficient arrays");
+  // cache policies of the coefficient arrays and of d / u
+  constexpr int CPC = CP == kMixed ? kStream : CP;
+  constexpr int CPD = CP == kMixed ? kCached : CP;
   constexpr bool STAGED = VEC == 2 || VEC == 3;
   constexpr bool GSTAGE = VEC == 4;
   // per-wave staging buffer (elements)
@@ -963,14 +971,14 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
     const int lane = tid & (kWave - 1);
     if (wave_active) {
       if constexpr (Coef::kArrays) {
-        int sh = gspan_load<T, R, I, CP>(coef.A, wave_off[0], wave_len, stage, lane);
+        int sh = gspan_load<T, R, I, CPC>(coef.A, wave_off[0], wave_len, stage, lane);
         gspan_take<T, R>(stage, sh + g_idx, g_rows, a, T(0));
-        sh = gspan_load<T, R, I, CP>(coef.B, wave_off[1], wave_len, stage, lane);
+        sh = gspan_load<T, R, I, CPC>(coef.B, wave_off[1], wave_len, stage, lane);
         gspan_take<T, R>(stage, sh + g_idx, g_rows, b, T(1));
-        sh = gspan_load<T, R, I, CP>(coef.C, wave_off[2], wave_len, stage, lane);
+        sh = gspan_load<T, R, I, CPC>(coef.C, wave_off[2], wave_len, stage, lane);
         gspan_take<T, R>(stage, sh + g_idx, g_rows, c, T(0));
       }
-      const int sh = gspan_load<T, R, I, CP>(Dd, wave_off[3], wave_len, stage, lane);
+      const int sh = gspan_load<T, R, I, CPD>(Dd, wave_off[3], wave_len, stage, lane);
       gspan_take<T, R>(stage, sh + g_idx, g_rows, d, T(0));
     } else {
 #pragma unroll
@@ -990,11 +998,11 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
     const int lane = tid & (kWave - 1);
     if (wave_active) {
       if constexpr (Coef::kArrays) {
-        load_chunk_staged<T, R, CP>(at(coef.A, wave_off[0]), stage, lane, a);
-        load_chunk_staged<T, R, CP>(at(coef.B, wave_off[1]), stage, lane, b);
-        load_chunk_staged<T, R, CP>(at(coef.C, wave_off[2]), stage, lane, c);
+        load_chunk_staged<T, R, CPC>(at(coef.A, wave_off[0]), stage, lane, a);
+        load_chunk_staged<T, R, CPC>(at(coef.B, wave_off[1]), stage, lane, b);
+        load_chunk_staged<T, R, CPC>(at(coef.C, wave_off[2]), stage, lane, c);
       }
-      load_chunk_staged<T, R, CP>(at(Dd, wave_off[3]), stage, lane, d);
+      load_chunk_staged<T, R, CPD>(at(Dd, wave_off[3]), stage, lane, d);
     } else {
 #pragma unroll
       for (int r = 0; r < R; r++) {
@@ -1010,20 +1018,20 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
     // (staged layouts never get here: VEC is 0 or 1)
     if constexpr (VEC == 1) {
       if constexpr (Coef::kArrays) {
-        load_chunk_vec<T, R, CP>(at(coef.A, oa), a);
-        load_chunk_vec<T, R, CP>(at(coef.B, ob), b);
-        load_chunk_vec<T, R, CP>(at(coef.C, oc), c);
+        load_chunk_vec<T, R, CPC>(at(coef.A, oa), a);
+        load_chunk_vec<T, R, CPC>(at(coef.B, ob), b);
+        load_chunk_vec<T, R, CPC>(at(coef.C, oc), c);
       }
-      load_chunk_vec<T, R, CP>(at(Dd, od), d);
+      load_chunk_vec<T, R, CPD>(at(Dd, od), d);
     } else {
 #pragma unroll
       for (int r = 0; r < R; r++) {
         if constexpr (Coef::kArrays) {
-          a[r] = ld_stream<CP>(at(coef.A, oa));
-          b[r] = ld_stream<CP>(at(coef.B, ob));
-          c[r] = ld_stream<CP>(at(coef.C, oc));
+          a[r] = ld_stream<CPC>(at(coef.A, oa));
+          b[r] = ld_stream<CPC>(at(coef.B, ob));
+          c[r] = ld_stream<CPC>(at(coef.C, oc));
         }
-        d[r] = ld_stream<CP>(at(Dd, od));
+        d[r] = ld_stream<CPD>(at(Dd, od));
         oa += s0;
         ob += s1;
         oc += s2;
@@ -1036,11 +1044,11 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
       const int row = row0 + r;
       if (active && row < N) {
         if constexpr (Coef::kArrays) {
-          a[r] = row == 0 ? T(0) : ld_stream<CP>(at(coef.A, off[0] + (I)row * s0));
-          b[r] = ld_stream<CP>(at(coef.B, off[1] + (I)row * s1));
-          c[r] = row == N - 1 ? T(0) : ld_stream<CP>(at(coef.C, off[2] + (I)row * s2));
+          a[r] = row == 0 ? T(0) : ld_stream<CPC>(at(coef.A, off[0] + (I)row * s0));
+          b[r] = ld_stream<CPC>(at(coef.B, off[1] + (I)row * s1));
+          c[r] = row == N - 1 ? T(0) : ld_stream<CPC>(at(coef.C, off[2] + (I)row * s2));
         }
-        d[r] = ld_stream<CP>(at(Dd, off[3] + (I)row * s3));
+        d[r] = ld_stream<CPD>(at(Dd, off[3] + (I)row * s3));
       } else {
         a[r] = T(0);
         b[r] = T(1);
@@ -1181,44 +1189,44 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
         if (r < g_rows) stage[sh + g_idx + r] = y[r];
       __builtin_amdgcn_wave_barrier();
       if (INC)
-        gspan_store<T, R, I, CP>(Udst, wave_off[4], wave_len, stage, lane, true,
+        gspan_store<T, R, I, CPD>(Udst, wave_off[4], wave_len, stage, lane, true,
                              static_cast<const T *>(U));
       else
-        gspan_store<T, R, I, CP>(Dst, wave_off[3], wave_len, stage, lane, false);
+        gspan_store<T, R, I, CPD>(Dst, wave_off[3], wave_len, stage, lane, false);
     }
   } else if (STAGED) {
     const int lane = tid & (kWave - 1);
     if (wave_active) {
       if (INC)
-        store_chunk_staged<T, R, CP>(at(Udst, wave_off[4]), stage, lane, y, true,
+        store_chunk_staged<T, R, CPD>(at(Udst, wave_off[4]), stage, lane, y, true,
                                  at(static_cast<const T *>(U), wave_off[4]));
       else
-        store_chunk_staged<T, R, CP>(at(Dst, wave_off[3]), stage, lane, y, false);
+        store_chunk_staged<T, R, CPD>(at(Dst, wave_off[3]), stage, lane, y, false);
     }
   } else if (full) {
     if (INC) {
       I ou = off[4] + (I)row0 * s4;
       if constexpr (VEC == 1) {
         T u[R];
-        load_chunk_vec<T, R, CP>(at(U, ou), u);
+        load_chunk_vec<T, R, CPD>(at(U, ou), u);
 #pragma unroll
         for (int r = 0; r < R; r++) u[r] += y[r];
-        store_chunk_vec<T, R, CP>(at(Udst, ou), u);
+        store_chunk_vec<T, R, CPD>(at(Udst, ou), u);
       } else {
 #pragma unroll
         for (int r = 0; r < R; r++) {
-          st_stream<CP>(at(Udst, ou), ld_stream<CP>(at(U, ou)) + y[r]);
+          st_stream<CPD>(at(Udst, ou), ld_stream<CPD>(at(U, ou)) + y[r]);
           ou += s4;
         }
       }
     } else {
       I od = off[3] + (I)row0 * s3;
       if constexpr (VEC == 1) {
-        store_chunk_vec<T, R, CP>(at(Dst, od), y);
+        store_chunk_vec<T, R, CPD>(at(Dst, od), y);
       } else {
 #pragma unroll
         for (int r = 0; r < R; r++) {
-          st_stream<CP>(at(Dst, od), y[r]);
+          st_stream<CPD>(at(Dst, od), y[r]);
           od += s3;
         }
       }
@@ -1230,9 +1238,9 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
       if (row < N) {
         if (INC) {
           const I ou = off[4] + (I)row * s4;
-          st_stream<CP>(at(Udst, ou), ld_stream<CP>(at(U, ou)) + y[r]);
+          st_stream<CPD>(at(Udst, ou), ld_stream<CPD>(at(U, ou)) + y[r]);
         } else {
-          st_stream<CP>(at(Dst, off[3] + (I)row * s3), y[r]);
+          st_stream<CPD>(at(Dst, off[3] + (I)row * s3), y[r]);
         }
       }
     }

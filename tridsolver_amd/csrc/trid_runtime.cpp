@@ -128,10 +128,11 @@ bool wide_index() { return g_wide.load() != 0; }
 
 int forced_cache_policy() { return g_cache_policy.load(); }
 
-int cache_policy(size_t bytes) {
+int cache_policy(size_t bytes, size_t rhs_bytes) {
   const int f = g_cache_policy.load();
   if (f >= 0) return f;
-  return bytes <= kMallFitBytes ? 1 : 0;
+  if (bytes <= kMallFitBytes) return 1;
+  return rhs_bytes <= kMallFitBytes ? 2 : 0;
 }
 
 ForcedConfig forced_config() {
@@ -206,8 +207,8 @@ EXTERN_C tridStatus_t tridForceWideIndex(int on) {
 }
 
 EXTERN_C tridStatus_t tridForceCachePolicy(int policy) {
-  if (policy < -1 || policy > 1)
-    return fail(TRID_STATUS_INVALID_VALUE, "tridForceCachePolicy: policy must be -1, 0 or 1");
+  if (policy < -1 || policy > 2)
+    return fail(TRID_STATUS_INVALID_VALUE, "tridForceCachePolicy: policy must be -1, 0, 1 or 2");
   g_cache_policy.store(policy);
   return TRID_STATUS_SUCCESS;
 }

@@ -62,14 +62,17 @@ inline hipError_t launch_kernel(F kernel, dim3 grid, dim3 block, std::uint32_t l
   return hipGetLastError();
 }
 
-// Cache policy of the streaming accesses (0 non-temporal, 1 ordinary; the
-// kernels' kStream / kCached) for a solve, or a chain of solves such as an
-// ADI step, whose arrays span `bytes`: ordinary accesses when they fit in
-// about 1.5x MI355X's 256 MB last-level cache (kMallFitBytes), so that each
-// kernel finds much of what the previous one (or call) wrote there;
-// non-temporal past it.  tridForceCachePolicy overrides the rule.
+// Cache policy of the streaming accesses (0 non-temporal, 1 ordinary, 2
+// non-temporal coefficient arrays with ordinary d and u; the kernels'
+// kStream / kCached / kMixed) for a solve, or a chain of solves such as an
+// ADI step, whose arrays span `bytes`, of which `rhs_bytes` are d and u:
+// ordinary accesses when all of them fit in about 1.5x MI355X's 256 MB
+// last-level cache (kMallFitBytes), so that each kernel finds much of what
+// the previous one (or call) wrote there; else the mixed policy when d and u
+// fit (the coefficients stream past the cache and d stays in it);
+// non-temporal past that.  tridForceCachePolicy overrides the rule.
 constexpr size_t kMallFitBytes = 384ull << 20;
-int cache_policy(size_t bytes);
+int cache_policy(size_t bytes, size_t rhs_bytes = ~size_t(0));
 // The policy tridForceCachePolicy fixed, -1 when none is.
 int forced_cache_policy();
 

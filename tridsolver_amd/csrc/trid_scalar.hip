@@ -311,8 +311,9 @@ tridStatus_t run_cfg(const PartCfg &cfg, const T *a, const T *b, const T *c,
 }
 
 // cp: the cache policy the candidates run with; unless a policy is forced
-// (tridForceCachePolicy), the winner is then timed against its twin under
-// the other policy, which it keeps unless the twin is more than 2 % faster.
+// (tridForceCachePolicy), the winner is then timed against its twins under
+// the other policies, and keeps its own unless a twin is more than 2 %
+// faster.
 template <typename T, int INC>
 PartCfg autotune(int order, int vec, bool gvec, int idx32, bool dense, const T *a,
                  const T *b, const T *c, T *d, T *u, const SysDesc &D,
@@ -351,16 +352,21 @@ PartCfg autotune(int order, int vec, bool gvec, int idx32, bool dense, const T *
       fprintf(stderr, "[tridhip tune] N=%d nsys=%lld order %d %s: %.4f ms\n", D.N, D.nsys,
               order, cfg_name(cands[c], sizeof(T)).c_str(), tm[c]);
   if (i >= 0 && idx32 && forced_cache_policy() < 0) {
-    // the same layout under the other cache policy
-    PartCfg twin = best;
-    twin.cp = best.cp == kCached ? kStream : kCached;
-    cands.assign({best, twin});
-    const int j = pick_fastest(2, prep, run, st, &tm);
+    // the same layout under the other cache policies
+    const PartCfg win = best;
+    cands.assign({win});
+    for (int p : {kCached, kMixed, kStream})
+      if (p != win.cp) {
+        PartCfg twin = win;
+        twin.cp = p;
+        cands.push_back(twin);
+      }
+    const int j = pick_fastest((int)cands.size(), prep, run, st, &tm);
     if (j >= 0) best = cands[j];
     if (log && log[0] == '1')
-      fprintf(stderr, "[tridhip tune] N=%d nsys=%lld order %d cache policy: %s %.4f ms, %s %.4f ms\n",
-              D.N, D.nsys, order, cfg_name(cands[0], sizeof(T)).c_str(), tm[0],
-              cfg_name(cands[1], sizeof(T)).c_str(), tm[1]);
+      for (size_t c = 0; c < cands.size(); c++)
+        fprintf(stderr, "[tridhip tune] N=%d nsys=%lld order %d cache policy: %s %.4f ms\n",
+                D.N, D.nsys, order, cfg_name(cands[c], sizeof(T)).c_str(), tm[c]);
   }
   (void)hipMemcpyAsync(dst, saved, bytes, hipMemcpyDeviceToDevice, st);
   return best;
@@ -443,14 +449,15 @@ tridStatus_t solve(const T *a, const int *a_pads, const T *b,
   D.small_index = D.nsys < (1LL << 31) && !wide_index();
   // 32-bit offsets when every array (its allocated extent) fits in 4 GiB
   int idx32 = !wide_index();
-  size_t footprint = 0;  // bytes spanned by the arrays the solve touches
+  size_t footprint = 0, rhs_bytes = 0;  // bytes spanned by all arrays, by d and u
   for (int t = 0; t < 5; t++) {
     if (t == 4 && !INC) break;
     const unsigned long long bytes = (unsigned long long)extent(ndim, dims, pads[t]) * sizeof(T);
     if (bytes >= (1ULL << 32)) idx32 = 0;
     footprint += bytes;
+    if (t >= 3) rhs_bytes += bytes;
   }
-  const int cp_rule = cache_policy(footprint);
+  const int cp_rule = cache_policy(footprint, rhs_bytes);
 
   const int order = solvedim == 0 ? 0 : 1;
   // vector loads along the system need unit stride and 16-byte aligned chunks

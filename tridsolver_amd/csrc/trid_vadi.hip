@@ -430,7 +430,8 @@ SysDesc line_desc(const Grid &g, int axis) {
 // their adjoints, the gradients).
 int vadi_policy(const Grid &g) {
   if ((unsigned long long)g.n * 8 >= (1ULL << 32) || wide_index()) return kStream;
-  return cache_policy(10 * (size_t)g.n * sizeof(double));
+  const int p = cache_policy(10 * (size_t)g.n * sizeof(double));
+  return p == kMixed ? kCached : p;  // no coefficient arrays: d and u cached
 }
 
 // Generic kappa sweep along `axis` (partitioned Thomas, scalar loads):
