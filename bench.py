@@ -108,7 +108,8 @@ def kernel_of_config(cfg: str, dtype: str, inc: int, coef: str):
                 f"{coef}<{T}>, {cp}>")
     m = re.search(r"rhs\+x\(fused,R(\d+)((?:,\w+)*)\)", cfg)
     if m:
-        return f"k_adi_rhs_x<{T}, {m.group(1)}, {int('cached' in m.group(2).split(','))}>"
+        fl = m.group(2).split(",")
+        return f"k_adi_rhs_x<{T}, {m.group(1)}, {int('cached' in fl)}, {int('cc' in fl)}>"
     return None
 
 

@@ -72,9 +72,10 @@ EXTERN_C const char *tridAdiPlanDescribe(tridAdiPlan_t plan);
  * two adjacent lines per lane where the arrays allow it; xcd = 1: every XCD
  * takes one contiguous range of lines); rows = 0 the partitioned sweep
  * without precomputed factors.  dir = 0 pins the fused right-hand side + x
- * stage of a plan that has one: rows = 4, 8 or 16 (chunk length; pair = 0),
- * xcd = 1 for the XCD-contiguous block order (each XCD sweeps one range of
- * planes, so the stencil's neighbour planes hit its own L2).  Returns TRID_STATUS_INVALID_VALUE when the
+ * stage of a plan that has one: rows = 4, 8 or 16 (chunk length), pair = 1
+ * for the x sweep with precomputed factors (0: factors computed in the
+ * kernel), xcd = 1 for the XCD-contiguous block order (each XCD sweeps one
+ * range of planes, so the stencil's neighbour planes hit its own L2).  Returns TRID_STATUS_INVALID_VALUE when the
  * layout does not fit the line length.  Pending autotuning of the plan is
  * cancelled; a graph captured earlier keeps the kernels it was captured
  * with. */

@@ -158,8 +158,10 @@ def test_adi_every_sweep_layout(shape, dt):
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 def test_adi_every_fused_x_layout(shape, dt):
     """The fused right-hand side + x stage pinned (tridAdiPlanSetSweep dir 0)
-    to every chunk length and both block orders (the XCD-contiguous order
-    included), against the oracle; rows that do not fit are rejected."""
+    to every chunk length, both block orders (the XCD-contiguous order
+    included) and both forms of the x sweep (precomputed factors, pair = 1,
+    or factors computed in the kernel), against the oracle; rows that do not
+    fit are rejected."""
     import torch
     nx, ny, nz = shape
     u0 = rand_init(nx, ny, nz, dt, seed=nx + ny)
@@ -168,22 +170,24 @@ def test_adi_every_fused_x_layout(shape, dt):
     ran = 0
     for rows in (4, 8, 16):
         for xcd in (0, 1):
-            plan = AdiPlan(nx, ny, nz, torch_dt(dt), 1.0, 1)
-            assert "fused" in plan.describe()
-            cps = nx // rows
-            fits = nx % rows == 0 and cps <= 64 and ny % (64 // cps) == 0
-            if not fits:
-                with pytest.raises(ta.TridError):
-                    plan.set_sweep(0, rows, 0, xcd)
-                continue
-            plan.set_sweep(0, rows, 0, xcd)
-            desc = plan.describe().replace(",cached", "")  # the cache policy is not the layout
-            assert f"fused,R{rows}{',xcd' if xcd else ''})" in desc, plan.describe()
-            u = torch.from_numpy(u0.copy()).to("cuda")
-            plan.step(u, torch.empty_like(u), 3)
-            err = oracle.rel_err(exp, u.cpu().numpy().ravel())
-            assert err <= tol(dt), f"{shape} R{rows} xcd {xcd} [{plan.describe()}]: {err:.2e}"
-            ran += 1
+            for cc in (0, 1):
+                plan = AdiPlan(nx, ny, nz, torch_dt(dt), 1.0, 1)
+                assert "fused" in plan.describe()
+                cps = nx // rows
+                fits = nx % rows == 0 and cps <= 64 and ny % (64 // cps) == 0
+                if not fits:
+                    with pytest.raises(ta.TridError):
+                        plan.set_sweep(0, rows, cc, xcd)
+                    continue
+                plan.set_sweep(0, rows, cc, xcd)
+                desc = plan.describe().replace(",cached", "")  # the cache policy is not the layout
+                tag = f"fused,R{rows}{',cc' if cc else ''}{',xcd' if xcd else ''})"
+                assert tag in desc, plan.describe()
+                u = torch.from_numpy(u0.copy()).to("cuda")
+                plan.step(u, torch.empty_like(u), 3)
+                err = oracle.rel_err(exp, u.cpu().numpy().ravel())
+                assert err <= tol(dt), f"{shape} R{rows} xcd {xcd} cc {cc} [{plan.describe()}]: {err:.2e}"
+                ran += 1
     assert ran >= 1
 
 

@@ -51,7 +51,9 @@ class AdiPlan:
 
     def set_sweep(self, direction: int, rows: int, pair: int = 0, xcd: int = 0) -> None:
         """Pins the y (1) or z (2) sweep: rows 8/16/32 = precomputed factors
-        with that chunk length, 0 = the partitioned sweep."""
+        with that chunk length, 0 = the partitioned sweep.  Direction 0 pins
+        the fused right-hand side + x stage: rows 4/8/16, pair = 1 for
+        precomputed factors of the x sweep."""
         _check(lib().tridAdiPlanSetSweep(self._h, int(direction), int(rows), int(pair), int(xcd)),
                "tridAdiPlanSetSweep")
 

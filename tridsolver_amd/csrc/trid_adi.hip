@@ -96,90 +96,155 @@ __device__ __forceinline__ T rhs_point(T l, T r, T ym, T yp, T zm, T zp, T c,
 // grid rounded down to a multiple of 8), so that each XCD -- work-groups are
 // dealt to the 8 XCDs round-robin -- sweeps one contiguous range of planes
 // and finds the neighbour planes of its lines in its own L2.
-template <typename T, int R, int CP>
+// FR = 1 (a tuner candidate; it ties the in-kernel factors at 256^3 and loses
+// at 512^3, where its larger LDS footprint lowers occupancy, since the
+// kernel's time is set by its loads and transposes rather than its
+// arithmetic): the x sweep runs with precomputed factors F (k_cc_setup<T, R> of
+// the interior-line matrix for N = nx and these cps chunks; record of chunk
+// k at F + k * nf): the chunk sweep and the in-wave reduced solve touch d
+// only, with no reciprocals, and the reduced solve exchanges one value per
+// level instead of three.  Lines off the grid's interior have d = 0, so the
+// interior factors give them their (zero) solution too.  The work-group
+// copies the table into LDS transposed (factor i of chunk k at i * cps + k),
+// so a wave's factor reads are consecutive words.  CP: cache policy of the
+// du stores.
+template <typename T, int R, int CP, int FR>
 __global__ void __launch_bounds__(256)
     k_adi_rhs_x(const T *__restrict__ u, T *__restrict__ du, int nx, int ny,
-                int nz, T lam, CoefConst<T> cf, int cps_log2, int xcd) {
+                int nz, T lam, CoefConst<T> cf, int cps_log2, int xcd,
+                const T *__restrict__ F) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int tid = threadIdx.x, lane = tid & (kWave - 1);
   const int cps = 1 << cps_log2;
   const int k = tid & (cps - 1);
+  using Ly = CCLayout<R>;
+  const T *Ft = reinterpret_cast<const T *>(smem_raw) + 256 * R;
+  if constexpr (FR) {
+    const int nf = Ly::nf(cps_log2);
+    T *ft = reinterpret_cast<T *>(smem_raw) + 256 * R;
+    for (int i = tid; i < nf * cps; i += 256) ft[i] = F[(i & (cps - 1)) * nf + (i >> cps_log2)];
+    __syncthreads();
+  }
   const int lines_per_block = 256 >> cps_log2;
   const long long nlines = (long long)ny * nz;
-  long long blk = blockIdx.x;
-  if (xcd) {
-    const unsigned nb8 = gridDim.x & ~7u;
-    if (blockIdx.x < nb8) blk = (long long)(blockIdx.x & 7u) * (nb8 >> 3) + (blockIdx.x >> 3);
-  }
-  const long long line = blk * lines_per_block + (tid >> cps_log2);
-  const long long wline0 = blk * lines_per_block + ((tid & ~(kWave - 1)) >> cps_log2);
-  const bool wave_active = wline0 < nlines;
-  const long long nxy = (long long)nx * ny;
-  const int kw = (int)(wline0 / ny);
-  const bool plane_in = kw >= 1 && kw <= nz - 2;
-  const int jl = (int)(line % ny);
-  const bool line_in = plane_in && jl >= 1 && jl <= ny - 2;
-  T *stage = reinterpret_cast<T *>(smem_raw) + (tid >> 6) * (kWave * R);
-  const long long wo = wline0 * nx;
-  const int row0 = k * R;
-
-  T a[R], b[R], c[R], d[R];
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    const int row = row0 + r;
-    const bool in = line_in && row >= 1 && row + 2 <= nx;
-    a[r] = in ? cf.a : T(0);
-    b[r] = in ? cf.b : T(1);
-    c[r] = in ? cf.c : T(0);
-    d[r] = T(0);
-  }
-  if (wave_active && plane_in) {
-    // all five spans in flight before the first transpose
-    Span<T, R> sc, sym, syp, szm, szp;
-    span_issue<T, R>(u + wo, lane, sc);
-    span_issue<T, R>(u + wo - nx, lane, sym);
-    span_issue<T, R>(u + wo + nx, lane, syp);
-    span_issue<T, R>(u + wo - nxy, lane, szm);
-    span_issue<T, R>(u + wo + nxy, lane, szp);
-    // the stencil is evaluated in the loaded layout (lane holds elements
-    // (v*64 + lane)*E .. +E-1 of the wave's span): the neighbour spans line
-    // up element for element, the x-neighbours at piece ends come from the
-    // adjacent lanes; only the right-hand side is transposed into chunks
-    constexpr int E = Vec16<T>::n, NP = R / E;
-    const int nx_log2 = __builtin_ctz(nx);  // nx = cps * R, a power of two
-    const int jw = (int)(wline0 % ny);      // the wave's first line in its plane
-    Span<T, R> sd;
-#pragma unroll
-    for (int v = 0; v < NP; v++) {
-      const auto cv = sc.v[v];
-      T left = lane_read(cv[E - 1], lane > 0 ? lane - 1 : 0);
-      T right = lane_read(cv[0], lane < kWave - 1 ? lane + 1 : kWave - 1);
-      const T prev_last = v > 0 ? lane_bcast(sc.v[v > 0 ? v - 1 : 0][E - 1], kWave - 1) : T(0);
-      const T next_first = v + 1 < NP ? lane_bcast(sc.v[v + 1 < NP ? v + 1 : 0][0], 0) : T(0);
-      if (lane == 0) left = prev_last;
-      if (lane == kWave - 1) right = next_first;
-#pragma unroll
-      for (int j = 0; j < E; j++) {
-        const int e = (v * kWave + lane) * E + j;
-        const int row = e & (nx - 1);
-        const int jl_e = jw + (e >> nx_log2);
-        const bool in = jl_e >= 1 && jl_e <= ny - 2 && row >= 1 && row + 2 <= nx;
-        const T l = j == 0 ? left : cv[j > 0 ? j - 1 : 0];
-        const T rr = j == E - 1 ? right : cv[j + 1 < E ? j + 1 : 0];
-        sd.v[v][j] = in ? rhs_point(l, rr, sym.v[v][j], syp.v[v][j], szm.v[v][j], szp.v[v][j],
-                                    cv[j], lam)
-                        : T(0);
-      }
+  const long long nblk = (nlines + lines_per_block - 1) / lines_per_block;
+  // one pass per block of lines; a grid smaller than nblk (the persistent
+  // launch of FR = 1, which copies the factor table once) strides over them
+  for (long long b0 = blockIdx.x; b0 < nblk; b0 += gridDim.x) {
+    long long blk = b0;
+    if (xcd) {
+      const long long nb8 = nblk & ~7LL;
+      if (b0 < nb8) blk = (b0 & 7) * (nb8 >> 3) + (b0 >> 3);
     }
-    span_stage<T, R>(sd, stage, lane, d);
+    const long long line = blk * lines_per_block + (tid >> cps_log2);
+    const long long wline0 = blk * lines_per_block + ((tid & ~(kWave - 1)) >> cps_log2);
+    const bool wave_active = wline0 < nlines;
+    const long long nxy = (long long)nx * ny;
+    const int kw = (int)(wline0 / ny);
+    const bool plane_in = kw >= 1 && kw <= nz - 2;
+    const int jl = (int)(line % ny);
+    const bool line_in = plane_in && jl >= 1 && jl <= ny - 2;
+    T *stage = reinterpret_cast<T *>(smem_raw) + (tid >> 6) * (kWave * R);
+    const long long wo = wline0 * nx;
+    const int row0 = k * R;
+
+    T d[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) d[r] = T(0);
+    if (wave_active && plane_in) {
+      // all five spans in flight before the first transpose
+      Span<T, R> sc, sym, syp, szm, szp;
+      span_issue<T, R>(u + wo, lane, sc);
+      span_issue<T, R>(u + wo - nx, lane, sym);
+      span_issue<T, R>(u + wo + nx, lane, syp);
+      span_issue<T, R>(u + wo - nxy, lane, szm);
+      span_issue<T, R>(u + wo + nxy, lane, szp);
+      // the stencil is evaluated in the loaded layout (lane holds elements
+      // (v*64 + lane)*E .. +E-1 of the wave's span): the neighbour spans line
+      // up element for element, the x-neighbours at piece ends come from the
+      // adjacent lanes; only the right-hand side is transposed into chunks
+      constexpr int E = Vec16<T>::n, NP = R / E;
+      const int nx_log2 = __builtin_ctz(nx);  // nx = cps * R, a power of two
+      const int jw = (int)(wline0 % ny);      // the wave's first line in its plane
+      Span<T, R> sd;
+#pragma unroll
+      for (int v = 0; v < NP; v++) {
+        const auto cv = sc.v[v];
+        T left = lane_read(cv[E - 1], lane > 0 ? lane - 1 : 0);
+        T right = lane_read(cv[0], lane < kWave - 1 ? lane + 1 : kWave - 1);
+        const T prev_last = v > 0 ? lane_bcast(sc.v[v > 0 ? v - 1 : 0][E - 1], kWave - 1) : T(0);
+        const T next_first = v + 1 < NP ? lane_bcast(sc.v[v + 1 < NP ? v + 1 : 0][0], 0) : T(0);
+        if (lane == 0) left = prev_last;
+        if (lane == kWave - 1) right = next_first;
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+          const int e = (v * kWave + lane) * E + j;
+          const int row = e & (nx - 1);
+          const int jl_e = jw + (e >> nx_log2);
+          const bool in = jl_e >= 1 && jl_e <= ny - 2 && row >= 1 && row + 2 <= nx;
+          const T l = j == 0 ? left : cv[j > 0 ? j - 1 : 0];
+          const T rr = j == E - 1 ? right : cv[j + 1 < E ? j + 1 : 0];
+          sd.v[v][j] = in ? rhs_point(l, rr, sym.v[v][j], syp.v[v][j], szm.v[v][j], szp.v[v][j],
+                                      cv[j], lam)
+                          : T(0);
+        }
+      }
+      span_stage<T, R>(sd, stage, lane, d);
+    }
+    T y[R];
+    if constexpr (FR) {
+      // factor i of this lane's chunk
+      auto fv = [&](int i) { return Ft[i * cps + k]; };
+      // chunk sweep (d only), the recurrences of k_cc_sweep
+      T dp = d[1] * fv(Ly::R1);
+      d[1] = dp;
+#pragma unroll
+      for (int i = 2; i < R; i++) {
+        dp = (d[i] - fv(Ly::A + i - 2) * dp) * fv(Ly::RI + i - 2);
+        d[i] = dp;
+      }
+      const T ld = d[R - 1];
+#pragma unroll
+      for (int i = R - 3; i >= 1; i--) d[i] = d[i] - fv(Ly::CB + i - 1) * d[i + 1];
+      const T fd = (d[0] - fv(Ly::C0) * d[1]) * fv(Ly::R0);
+      // reduced system over the line's chunks (consecutive lanes)
+      T fnd = lane_read(fd, k + 1 < cps ? lane + 1 : lane);
+      if (k + 1 >= cps) fnd = T(0);
+      T rd = (ld - fv(Ly::LA) * fd - fv(Ly::LC) * fnd) * fv(Ly::INV);
+      int j = 0;
+      for (int st = 1; st < cps; st <<= 1, j++) {
+        T md = lane_read(rd, k >= st ? lane - st : lane);
+        T pd = lane_read(rd, k + st < cps ? lane + st : lane);
+        if (k < st) md = T(0);
+        if (k + st >= cps) pd = T(0);
+        rd = (rd - fv(Ly::PCR + 3 * j) * md - fv(Ly::PCR + 3 * j + 1) * pd) *
+             fv(Ly::PCR + 3 * j + 2);
+      }
+      T xm = lane_read(rd, k > 0 ? lane - 1 : lane);
+      if (k == 0) xm = T(0);
+      const T xf = fd - fv(Ly::FA) * xm - fv(Ly::FC) * rd;
+      y[0] = xf;
+      y[R - 1] = rd;
+#pragma unroll
+      for (int i = 1; i < R - 1; i++) y[i] = d[i] - fv(Ly::EA + i - 1) * xf - fv(Ly::EC + i - 1) * rd;
+    } else {
+      T a[R], b[R], c[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int row = row0 + r;
+        const bool in = line_in && row >= 1 && row + 2 <= nx;
+        a[r] = in ? cf.a : T(0);
+        b[r] = in ? cf.b : T(1);
+        c[r] = in ? cf.c : T(0);
+      }
+      Row<T> f, l;
+      chunk_reduce<T, R>(a, b, c, d, f, l);
+      T xf, xl;
+      reduced_wave<T>(f, l, k, cps, lane, xf, xl);
+      chunk_expand<T, R>(a, c, d, xf, xl, y);
+    }
+    if (wave_active) store_chunk_staged<T, R, CP>(du + wo, stage, lane, y, false);
   }
-  Row<T> f, l;
-  chunk_reduce<T, R>(a, b, c, d, f, l);
-  T xf, xl;
-  reduced_wave<T>(f, l, k, cps, lane, xf, xl);
-  T y[R];
-  chunk_expand<T, R>(a, c, d, xf, xl, y);
-  if (wave_active) store_chunk_staged<T, R, CP>(du + wo, stage, lane, y, false);
 }
 
 namespace {
@@ -219,6 +284,8 @@ struct tridAdiPlan_st {
   int fused_R = 0;          // >0: fused RHS + x sweep with this chunk length
   int fused_cps_log2 = 0;
   int fused_xcd = 0;        // XCD-contiguous block order of the fused kernel
+  int fused_cc = 0;         // fused x sweep with precomputed factors
+  void *fused_F[3] = {nullptr, nullptr, nullptr};  // their tables, chunk length 4, 8, 16
   int cp = 0;               // cache policy of every stage's streaming accesses
   // y and z sweeps with precomputed factors (trid_cc.hpp): chunk length (0 =
   // the generic kernel), chunks per system and the factor table
@@ -248,25 +315,50 @@ CoefConst<T> coefs(const tridAdiPlan_st *p) {
   return CoefConst<T>{-lam, (T)1.0f + (T)2.0f * lam, -lam};
 }
 
+int fused_slot(int R) { return R == 4 ? 0 : (R == 8 ? 1 : 2); }
+
 template <typename T>
 hipError_t launch_fused(tridAdiPlan_st *p, T *u, T *du, hipStream_t st) {
   const CoefConst<T> cf = coefs<T>(p);
   const int lines_per_block = 256 >> p->fused_cps_log2;
   const long long nlines = (long long)p->ny * p->nz;
   const unsigned blocks = (unsigned)((nlines + lines_per_block - 1) / lines_per_block);
-  const size_t lds = 256 * (size_t)p->fused_R * sizeof(T);
-#define TRID_FUSED(RR, CC)                                                                   \
-  (void)launch_kernel(k_adi_rhs_x<T, RR, CC>, dim3(blocks), dim3(256), (std::uint32_t)lds, st, \
-                      u, du, p->nx, p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2,       \
-                      p->fused_xcd)
-#define TRID_FUSED_R(CC)                  \
-  if (p->fused_R == 8) TRID_FUSED(8, CC);  \
-  else if (p->fused_R == 4) TRID_FUSED(4, CC); \
-  else TRID_FUSED(16, CC);
+  size_t lds = 256 * (size_t)p->fused_R * sizeof(T);
+  const T *F = static_cast<const T *>(p->fused_F[fused_slot(p->fused_R)]);
+  const int fr = p->fused_cc && F;
+  if (fr) {
+    // the transposed factor table after the staging buffers
+    const int cl = p->fused_cps_log2;
+    const int nf = p->fused_R == 4 ? CCLayout<4>::nf(cl)
+                                   : (p->fused_R == 8 ? CCLayout<8>::nf(cl) : CCLayout<16>::nf(cl));
+    lds += (size_t)nf * (1u << cl) * sizeof(T);
+  }
+  // the factor-table form runs persistent: as many work-groups as are
+  // resident at once, each copying the table into LDS once
+#define TRID_FUSED(RR, CC, FF)                                                                \
+  do {                                                                                        \
+    auto kern = k_adi_rhs_x<T, RR, CC, FF>;                                                   \
+    unsigned g = blocks;                                                                      \
+    if (FF) g = std::min(g, resident_blocks(reinterpret_cast<const void *>(kern), 256, lds)); \
+    (void)launch_kernel(kern, dim3(g), dim3(256), (std::uint32_t)lds, st, u, du, p->nx,      \
+                        p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2, p->fused_xcd, F); \
+  } while (0)
+#define TRID_FUSED_R(CC, FF)                        \
+  if (p->fused_R == 8) TRID_FUSED(8, CC, FF);       \
+  else if (p->fused_R == 4) TRID_FUSED(4, CC, FF);  \
+  else TRID_FUSED(16, CC, FF);
   if (p->cp == kCached) {
-    TRID_FUSED_R(kCached)
+    if (fr) {
+      TRID_FUSED_R(kCached, 1)
+    } else {
+      TRID_FUSED_R(kCached, 0)
+    }
   } else {
-    TRID_FUSED_R(kStream)
+    if (fr) {
+      TRID_FUSED_R(kStream, 1)
+    } else {
+      TRID_FUSED_R(kStream, 0)
+    }
   }
 #undef TRID_FUSED_R
 #undef TRID_FUSED
@@ -323,6 +415,42 @@ tridStatus_t cc_build(tridAdiPlan_st *p, int dir, int R, void **F, int *cps_log2
     *F = nullptr;
   }
   return s;
+}
+
+// Factor table of the fused x sweep with chunk length R (fused_rows_ok(p, R)
+// holds: nx = R * cps, cps <= 64), built once per plan and chunk length into
+// p->fused_F (waits for the setup kernel).
+template <typename T>
+tridStatus_t fused_table(tridAdiPlan_st *p, int R) {
+  void *&F = p->fused_F[fused_slot(R)];
+  if (F) return TRID_STATUS_SUCCESS;
+  const int cl = ilog2(p->nx / R), cps = 1 << cl;
+  const size_t nf = R == 4 ? CCLayout<4>::nf(cl) : (R == 8 ? CCLayout<8>::nf(cl) : CCLayout<16>::nf(cl));
+  tridStatus_t s = check_hip(hipMalloc(&F, cps * nf * sizeof(T)), "hipMalloc");
+  if (s) {
+    F = nullptr;
+    return s;
+  }
+  const CoefConst<T> cf = coefs<T>(p);
+  hipStream_t st = current_stream();
+  T *f = static_cast<T *>(F);
+  if (R == 4)
+    hipLaunchKernelGGL((k_cc_setup<T, 4>), dim3(1), dim3(kWave), 0, st, cf, p->nx, cl, f);
+  else if (R == 8)
+    hipLaunchKernelGGL((k_cc_setup<T, 8>), dim3(1), dim3(kWave), 0, st, cf, p->nx, cl, f);
+  else
+    hipLaunchKernelGGL((k_cc_setup<T, 16>), dim3(1), dim3(kWave), 0, st, cf, p->nx, cl, f);
+  s = check_hip(hipGetLastError(), "factor setup launch");
+  if (!s) s = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+  if (s) {
+    (void)hipFree(F);
+    F = nullptr;
+  }
+  return s;
+}
+
+tridStatus_t fused_table_any(tridAdiPlan_st *p, int R) {
+  return p->dbl ? fused_table<double>(p, R) : fused_table<float>(p, R);
 }
 
 // Two adjacent systems per lane need the systems' first other dimension to
@@ -635,16 +763,28 @@ void tune_plan_t(tridAdiPlan_st *p, T *u, T *du) {
     for (void *F : tables)
       if (F != p->cc_F[dir]) (void)hipFree(F);
   }
-  // chunk length and block order of the fused RHS + x kernel (the untuned
-  // choice first)
+  // chunk length, block order and precomputed factors of the fused RHS + x
+  // kernel (the untuned choice first)
   if (p->fused_R) {
-    std::vector<std::pair<int, int>> ch{{p->fused_R, p->fused_xcd}};
-    for (int R : {4, 8, 16})
-      for (int x : {0, 1})
-        if (fused_rows_ok(p, R) && !(R == p->fused_R && x == p->fused_xcd)) ch.push_back({R, x});
+    struct FusedChoice {
+      int R, xcd, cc;
+    };
+    std::vector<FusedChoice> ch{{p->fused_R, p->fused_xcd, p->fused_cc}};
+    for (int cc : {1, 0})
+      for (int R : {4, 8, 16})
+        for (int x : {0, 1}) {
+          if (!fused_rows_ok(p, R)) continue;
+          if (R == p->fused_R && x == p->fused_xcd && cc == p->fused_cc) continue;
+          if (cc && fused_table<T>(p, R) != TRID_STATUS_SUCCESS) continue;
+          ch.push_back({R, x, cc});
+        }
+    auto apply = [&](const FusedChoice &c) {
+      set_fused(p, c.R);
+      p->fused_xcd = c.xcd;
+      p->fused_cc = c.cc;
+    };
     auto cand = [&](int i) {
-      set_fused(p, ch[i].first);
-      p->fused_xcd = ch[i].second;
+      apply(ch[i]);
       return launch_fused<T>(p, u, du, st);
     };
     auto suffix = [&]() {
@@ -655,10 +795,9 @@ void tune_plan_t(tridAdiPlan_st *p, T *u, T *du) {
     const int i = pick_in_step((int)ch.size(), []() {}, cand, suffix, st, &tm);
     if (tune_log())
       for (size_t c = 0; c < ch.size(); c++)
-        fprintf(stderr, "[tridhip tune] rhs+x fused R%d%s: %.4f ms\n", ch[c].first,
-                ch[c].second ? ",xcd" : "", tm[c]);
-    set_fused(p, ch[i >= 0 ? i : 0].first);
-    p->fused_xcd = ch[i >= 0 ? i : 0].second;
+        fprintf(stderr, "[tridhip tune] rhs+x fused R%d%s%s: %.4f ms\n", ch[c].R,
+                ch[c].cc ? ",cc" : "", ch[c].xcd ? ",xcd" : "", tm[c]);
+    apply(ch[i >= 0 ? i : 0]);
   }
   // cache policy of the whole step (the other policy changes what every
   // stage finds in the last-level cache, so whole steps are timed): the
@@ -704,8 +843,8 @@ std::string plan_text(const tridAdiPlan_st *p) {
   };
   std::string fx = "rhs" + cached + "<";
   if (p->fused_R)
-    fx = "rhs+x(fused,R" + std::to_string(p->fused_R) + (p->fused_xcd ? ",xcd" : "") + cached +
-         ")<";
+    fx = "rhs+x(fused,R" + std::to_string(p->fused_R) + (p->fused_cc ? ",cc" : "") +
+         (p->fused_xcd ? ",xcd" : "") + cached + ")<";
   return fx + (p->dbl ? "double" : "float") + "> x:" + part_name(0) + " y:" + sweep_name(1) +
          " z:" + sweep_name(2);
 }
@@ -877,6 +1016,8 @@ EXTERN_C tridStatus_t tridAdiPlanDestroy(tridAdiPlan_t p) {
   if (p->cap) (void)hipStreamDestroy(p->cap);
   for (void *F : p->cc_F)
     if (F) (void)hipFree(F);
+  for (void *F : p->fused_F)
+    if (F) (void)hipFree(F);
   delete p;
   return TRID_STATUS_SUCCESS;
 }
@@ -973,11 +1114,16 @@ EXTERN_C tridStatus_t tridAdiPlanSetSweep(tridAdiPlan_t p, int dir, int rows, in
   if (dir == 0) {
     // the fused right-hand side + x sweep: chunk length and block order
     if (!p->fused_R) return fail(TRID_STATUS_INVALID_VALUE, "the plan has no fused x stage");
-    if (pair || !fused_rows_ok(p, rows))
-      return fail(TRID_STATUS_INVALID_VALUE, "rows do not fit the fused x stage (pair = 0)");
+    if (!fused_rows_ok(p, rows))
+      return fail(TRID_STATUS_INVALID_VALUE, "rows do not fit the fused x stage");
+    if (pair) {
+      const tridStatus_t s = fused_table_any(p, rows);
+      if (s) return s;
+    }
     drop_graphs(p, true);
     set_fused(p, rows);
     p->fused_xcd = xcd;
+    p->fused_cc = pair;
     p->tune_pending = false;
     p->text = plan_text(p);
     clear_error();

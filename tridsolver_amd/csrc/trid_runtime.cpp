@@ -3,11 +3,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstdlib>
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -127,6 +129,27 @@ int autotune_mode() {
 bool wide_index() { return g_wide.load() != 0; }
 
 int forced_cache_policy() { return g_cache_policy.load(); }
+
+unsigned resident_blocks(const void *kernel, int threads, size_t lds) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, const void *, int, size_t>, unsigned> cache;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  const auto key = std::make_tuple(dev, kernel, threads, lds);
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    per_cu = 1;
+    cus = 256;
+  }
+  const unsigned n = (unsigned)std::max(1, per_cu) * (unsigned)std::max(1, cus);
+  cache[key] = n;
+  return n;
+}
 
 int cache_policy(size_t bytes, size_t rhs_bytes) {
   const int f = g_cache_policy.load();

@@ -76,6 +76,11 @@ int cache_policy(size_t bytes, size_t rhs_bytes = ~size_t(0));
 // The policy tridForceCachePolicy fixed, -1 when none is.
 int forced_cache_policy();
 
+// Work-groups of `threads` threads and `lds` bytes of dynamic LDS of
+// `kernel` that the current device holds at once (occupancy x compute
+// units; at least 1), cached per device and configuration.
+unsigned resident_blocks(const void *kernel, int threads, size_t lds);
+
 // True while tridForceWideIndex(1) is in effect: the kernels then use 64-bit
 // element offsets and 64-bit system-index arithmetic whatever the sizes.
 bool wide_index();
