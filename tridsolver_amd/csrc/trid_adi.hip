@@ -85,6 +85,60 @@ __device__ __forceinline__ T rhs_point(T l, T r, T ym, T yp, T zm, T zp, T c,
   return lam * s;
 }
 
+// Padded wave staging for the fused kernel: piece pc (16 bytes) of the
+// wave's span sits at slot pc + pc / P (P = R / E pieces per chunk, one pad
+// slot per chunk when P > 1), so lane l's chunk is the P slots from
+// l * (P + 1): with an odd slot stride the per-lane chunk accesses of
+// consecutive lanes fall in distinct LDS banks (a stride of P slots puts
+// lanes l and l + 16 / P in the same bank).
+template <typename T, int R>
+struct PadStage {
+  static constexpr int E = Vec16<T>::n, P = R / E, PAD = P > 1 ? 1 : 0;
+  // elements of one wave's buffer
+  static constexpr int elems = kWave * (P + PAD) * E;
+  __device__ static int slot(int pc) { return pc + (pc / P) * PAD; }
+};
+
+template <typename T, int R>
+__device__ __forceinline__ void span_stage_pad(const Span<T, R> &t, T *__restrict__ buf,
+                                               int lane, T (&x)[R]) {
+  using V = typename Vec16<T>::type;
+  using S = PadStage<T, R>;
+  V *s = reinterpret_cast<V *>(buf);
+#pragma unroll
+  for (int v = 0; v < S::P; v++) s[S::slot(v * kWave + lane)] = t.v[v];
+  __builtin_amdgcn_wave_barrier();
+  const V *mine = s + lane * (S::P + S::PAD);
+#pragma unroll
+  for (int v = 0; v < S::P; v++) {
+    const V w = mine[v];
+#pragma unroll
+    for (int e = 0; e < S::E; e++) x[S::E * v + e] = w[e];
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <typename T, int R, int CP>
+__device__ __forceinline__ void store_staged_pad(T *__restrict__ span, T *__restrict__ buf,
+                                                 int lane, const T (&x)[R]) {
+  using V = typename Vec16<T>::type;
+  using S = PadStage<T, R>;
+  V *s = reinterpret_cast<V *>(buf);
+  V *mine = s + lane * (S::P + S::PAD);
+#pragma unroll
+  for (int v = 0; v < S::P; v++) {
+    V w;
+#pragma unroll
+    for (int e = 0; e < S::E; e++) w[e] = x[S::E * v + e];
+    mine[v] = w;
+  }
+  __builtin_amdgcn_wave_barrier();
+  V *g = reinterpret_cast<V *>(span);
+#pragma unroll
+  for (int v = 0; v < S::P; v++) st_stream<CP>(g + v * kWave + lane, s[S::slot(v * kWave + lane)]);
+  __builtin_amdgcn_wave_barrier();
+}
+
 // Fused right-hand side + x sweep: du <- A_x^{-1} (lambda * Laplacian u).
 // One wave owns 64/cps whole x-lines of one z-plane (the host requires
 // nx == cps*R, cps <= 64 and ny a multiple of the lines per wave, so a wave
@@ -118,10 +172,12 @@ __global__ void __launch_bounds__(256)
   const int cps = 1 << cps_log2;
   const int k = tid & (cps - 1);
   using Ly = CCLayout<R>;
-  const T *Ft = reinterpret_cast<const T *>(smem_raw) + 256 * R;
+  // per-wave staging buffers, then (FR) the factor table
+  constexpr int SW = PadStage<T, R>::elems;
+  const T *Ft = reinterpret_cast<const T *>(smem_raw) + 4 * SW;
   if constexpr (FR) {
     const int nf = Ly::nf(cps_log2);
-    T *ft = reinterpret_cast<T *>(smem_raw) + 256 * R;
+    T *ft = reinterpret_cast<T *>(smem_raw) + 4 * SW;
     for (int i = tid; i < nf * cps; i += 256) ft[i] = F[(i & (cps - 1)) * nf + (i >> cps_log2)];
     __syncthreads();
   }
@@ -144,7 +200,7 @@ __global__ void __launch_bounds__(256)
     const bool plane_in = kw >= 1 && kw <= nz - 2;
     const int jl = (int)(line % ny);
     const bool line_in = plane_in && jl >= 1 && jl <= ny - 2;
-    T *stage = reinterpret_cast<T *>(smem_raw) + (tid >> 6) * (kWave * R);
+    T *stage = reinterpret_cast<T *>(smem_raw) + (tid >> 6) * SW;
     const long long wo = wline0 * nx;
     const int row0 = k * R;
 
@@ -189,7 +245,7 @@ __global__ void __launch_bounds__(256)
                           : T(0);
         }
       }
-      span_stage<T, R>(sd, stage, lane, d);
+      span_stage_pad<T, R>(sd, stage, lane, d);
     }
     T y[R];
     if constexpr (FR) {
@@ -243,7 +299,7 @@ __global__ void __launch_bounds__(256)
       reduced_wave<T>(f, l, k, cps, lane, xf, xl);
       chunk_expand<T, R>(a, c, d, xf, xl, y);
     }
-    if (wave_active) store_chunk_staged<T, R, CP>(du + wo, stage, lane, y, false);
+    if (wave_active) store_staged_pad<T, R, CP>(du + wo, stage, lane, y);
   }
 }
 
@@ -323,7 +379,11 @@ hipError_t launch_fused(tridAdiPlan_st *p, T *u, T *du, hipStream_t st) {
   const int lines_per_block = 256 >> p->fused_cps_log2;
   const long long nlines = (long long)p->ny * p->nz;
   const unsigned blocks = (unsigned)((nlines + lines_per_block - 1) / lines_per_block);
-  size_t lds = 256 * (size_t)p->fused_R * sizeof(T);
+  // four padded wave staging buffers
+  const int R = p->fused_R;
+  size_t lds = 4 * sizeof(T) *
+               (size_t)(R == 4 ? PadStage<T, 4>::elems
+                               : (R == 8 ? PadStage<T, 8>::elems : PadStage<T, 16>::elems));
   const T *F = static_cast<const T *>(p->fused_F[fused_slot(p->fused_R)]);
   const int fr = p->fused_cc && F;
   if (fr) {
