@@ -116,6 +116,39 @@ __device__ __forceinline__ int rag_load(const T *__restrict__ p, int len, T *__r
   return sh;
 }
 
+// rag_load for a span group of a whole wave (G = 64), straight into LDS:
+// global_load_lds moves each lane's 16-byte piece to reg + (round * 64 +
+// lane) pieces -- the layout rag_load builds -- without passing through
+// registers, so the loads of several arrays can be in flight at once.  The
+// caller waits (rag_lds_wait) before reading reg.  Non-temporal, as
+// ld_stream.  Returns sh.
+template <typename T, int R>
+__device__ __forceinline__ int rag_load_lds(const T *__restrict__ p, int len, T *__restrict__ reg,
+                                            int lane) {
+  using V = typename Vec16<T>::type;
+  constexpr int E = Vec16<T>::n;
+  const int sh = (int)((reinterpret_cast<uintptr_t>(p) / sizeof(T)) & (E - 1));
+  const V *g = reinterpret_cast<const V *>(p - sh);
+  const int np = len > 0 ? (sh + len + E - 1) / E : 0;
+  constexpr int NR = (R + E - 1) / E + 1;
+#pragma unroll
+  for (int j = 0; j < NR; j++) {
+    const int pc = j * kWave + lane;
+    if (pc < np)
+      __builtin_amdgcn_global_load_lds(
+          (const void *)(g + pc),
+          (__attribute__((address_space(3))) void *)(reg + j * kWave * E), 16, 0,
+          2 /* non-temporal */);
+  }
+  return sh;
+}
+
+// Waits for the wave's loads into LDS (and every earlier access).
+__device__ __forceinline__ void rag_lds_wait() {
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+}
+
 // The lane's rows i0 .. i0+R-1 of the span (pad past len) from reg.
 template <typename T, int R>
 __device__ __forceinline__ void rag_take(const T *reg, int sh, int i0, int len, T (&x)[R],
@@ -191,14 +224,33 @@ __device__ __forceinline__ void rag_item(const RagArgs<T> &g, const RagItem &it,
   const long long p0 = len > 0 ? off + row_base : 0;
   const int i0 = kg * R;  // the lane's first row within the span
   T a[R], b[R], c[R], d[R];
-  int sh = rag_load<T, R>(g.a + p0, len, reg, kg, G);
-  rag_take<T, R>(reg, sh, i0, len, a, T(0));
-  sh = rag_load<T, R>(g.b + p0, len, reg, kg, G);
-  rag_take<T, R>(reg, sh, i0, len, b, T(1));
-  sh = rag_load<T, R>(g.c + p0, len, reg, kg, G);
-  rag_take<T, R>(reg, sh, i0, len, c, T(0));
-  sh = rag_load<T, R>(g.d + p0, len, reg, kg, G);
-  rag_take<T, R>(reg, sh, i0, len, d, T(0));
+  if (G == kWave) {
+    // whole-wave span groups (systems of 64 chunks and more): two arrays
+    // in flight at a time, straight into LDS (the second array's regions
+    // follow the eight waves' first ones)
+    T *reg2 = smem + (kRagThreads / kWave) * (kWave * R + 2 * E) + (tid / kWave) * (kWave * R + 2 * E);
+    rag_lds_wait();  // the previous item's reads of these regions are done
+    int sa = rag_load_lds<T, R>(g.a + p0, len, reg, kg);
+    int sb = rag_load_lds<T, R>(g.b + p0, len, reg2, kg);
+    rag_lds_wait();
+    rag_take<T, R>(reg, sa, i0, len, a, T(0));
+    rag_take<T, R>(reg2, sb, i0, len, b, T(1));
+    rag_lds_wait();
+    sa = rag_load_lds<T, R>(g.c + p0, len, reg, kg);
+    sb = rag_load_lds<T, R>(g.d + p0, len, reg2, kg);
+    rag_lds_wait();
+    rag_take<T, R>(reg, sa, i0, len, c, T(0));
+    rag_take<T, R>(reg2, sb, i0, len, d, T(0));
+  } else {
+    int sh = rag_load<T, R>(g.a + p0, len, reg, kg, G);
+    rag_take<T, R>(reg, sh, i0, len, a, T(0));
+    sh = rag_load<T, R>(g.b + p0, len, reg, kg, G);
+    rag_take<T, R>(reg, sh, i0, len, b, T(1));
+    sh = rag_load<T, R>(g.c + p0, len, reg, kg, G);
+    rag_take<T, R>(reg, sh, i0, len, c, T(0));
+    sh = rag_load<T, R>(g.d + p0, len, reg, kg, G);
+    rag_take<T, R>(reg, sh, i0, len, d, T(0));
+  }
   const int row0 = k * R;
   if (row0 == 0) a[0] = T(0);
 #pragma unroll
@@ -484,9 +536,13 @@ void occupancy(size_t lds, int &per_cu) {
 }
 
 size_t family_lds(int f, size_t ts) {
-  // span-group regions (at most 512 groups of G*R + 2E elements, G >= 1);
+  // span-group regions (at most 512 groups of G*R + 2E elements, G >= 1),
+  // or, for whole-wave groups, two sets of eight wave regions of 64R + 2E;
   // the reduced solves reuse them
-  return (size_t)kRagThreads * (kRagRhi[f] + 2 * (16 / ts)) * ts;
+  const size_t E = 16 / ts, R = kRagRhi[f];
+  const size_t groups = (size_t)kRagThreads * (R + 2 * E);
+  const size_t waves2 = 2 * (size_t)(kRagThreads / kWave) * (kWave * R + 2 * E);
+  return std::max(groups, waves2) * ts;
 }
 
 int family_grid(int f, bool dbl, int nitems) {
