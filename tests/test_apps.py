@@ -5,8 +5,8 @@ compare_{d,s} (SumOfDiff of two raw ADI outputs) are checked
     built unmodified into oracle/_ref/ where the reference tree is present):
     identical standard output on the same files, byte for byte;
   * against the sequential sum of the differences in the file's precision.
-adi_hip_{d,s} need a GPU for a run (tests/test_gpu_apps.py); here only their
-option handling is checked.
+adi_hip_{d,s} and blktrid_hip_{d,s} need a GPU for a run
+(tests/test_gpu_apps.py); here only their option handling is checked.
 """
 from __future__ import annotations
 
@@ -28,8 +28,21 @@ def run(exe, *args):
 
 
 def test_tools_are_built():
-    for name in ("adi_hip_d", "adi_hip_s", "compare_d", "compare_s"):
+    for name in ("adi_hip_d", "adi_hip_s", "compare_d", "compare_s", "blktrid_hip_d",
+                 "blktrid_hip_s"):
         assert os.access(os.path.join(BIN, name), os.X_OK), name
+
+
+@pytest.mark.parametrize("tag", ["d", "s"])
+def test_blktrid_tool_options(tag):
+    """the reference block app's usage text for -help, and bad sizes refused
+    before any device work"""
+    exe = os.path.join(BIN, f"blktrid_hip_{tag}")
+    out = run(exe, "-help")
+    assert out.returncode == 0 and "-blkdim=BLOCKDIM" in out.stdout, out.stdout
+    for bad in (["-n=0"], ["-blkdim=11"], ["-p=-3"], ["-iter=0"]):
+        out = run(exe, *bad)
+        assert out.returncode == 1 and "invalid arguments" in out.stderr, (bad, out.stderr)
 
 
 def files(tmp_path, dt, nan_at=None, seed=0, n=16 * 12 * 10):

@@ -86,3 +86,59 @@ def test_adi_app_vs_reference_gold(tmp_path, name, meta, prof):
         num = np.array([not (s in ("k", "=") ) for s in w])
         wn, hn = w[num].astype(np.float64), h[num].astype(np.float64)
         assert np.allclose(hn, wn, rtol=1e-4, atol=1e-4)
+
+
+def splitmix64(z):
+    """S. Vigna's SplitMix64 on a numpy uint64 array"""
+    z = z + np.uint64(0x9e3779b97f4a7c15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xbf58476d1ce4e5b9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94d049bb133111eb)
+    return z ^ (z >> np.uint64(31))
+
+
+def blktrid_system(n, p, m):
+    """the tool's generated system in the GPU layout (FP64)"""
+    A = np.zeros((n, p, m, m))
+    C = np.zeros((n, p, m, m))
+    B = np.zeros((n, p, m, m))
+    idx = np.arange(m)
+    A[:, :, idx, idx] = 0.01
+    C[:, :, idx, idx] = 0.01
+    B[:, :, idx[1:], idx[:-1]] = 0.01
+    B[:, :, idx[:-1], idx[1:]] = 0.01
+    k = (np.arange(n * p, dtype=np.uint64)[:, None] * np.uint64(m) + idx.astype(np.uint64)[None, :])
+    with np.errstate(over="ignore"):
+        r = splitmix64(np.uint64(10) + k)
+    B[:, :, idx, idx] = (4.0 + 0.01 * ((r >> np.uint64(11)).astype(np.float64) * 2.0 ** -53)).reshape(n, p, m)
+    d = np.ones((n, p, m))
+    return A.ravel(), B.ravel(), C.ravel(), d.ravel()
+
+
+@pytest.mark.parametrize("tag", ["d", "s"])
+@pytest.mark.parametrize("m", [2, 3, 5, 8])
+def test_blktrid_tool(tmp_path, tag, m):
+    """blktrid_hip_{d,s}: the reference block app's report, a relative
+    residual at the precision's level, and results_binary (the solution in
+    the GPU layout) against the oracle's block Thomas (FP64) on the same
+    system, rebuilt here from the tool's generator (1e-12 FP64, 1e-5 FP32
+    normwise relative)."""
+    import oracle
+    n, p = 12, 300
+    exe = os.path.join(BIN, f"blktrid_hip_{tag}")
+    print(f"blktrid_hip_{tag} n={n} p={p} blkdim={m}", flush=True)
+    out = subprocess.run([exe, f"-n={n}", f"-p={p}", f"-blkdim={m}", "-iter=2"], cwd=tmp_path,
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert f"N:\t{n}\nP:\t{p}\nblkdim:\t{m}" in out.stdout
+    assert re.search(r"GPU \n  duration \[ms\]: [0-9.]+\n  GFLOPS: +[0-9.]+\n  GB/sec: +[0-9.]+",
+                     out.stdout), out.stdout
+    res = float(re.search(r"Relative residual \|\|A u - d\|\| / \|\|d\|\|: (\S+)",
+                          out.stdout).group(1))
+    assert res <= (1e-13 if tag == "d" else 1e-6), res
+    dt = np.float64 if tag == "d" else np.float32
+    u = np.fromfile(tmp_path / "results_binary", dtype=dt)
+    assert u.size == n * p * m
+    A, B, C, d = blktrid_system(n, p, m)
+    exp = oracle.btsv(A, B, C, d, n, p, m, layout=0)
+    err = oracle.rel_err(exp, u.astype(np.float64))
+    assert err <= (1e-12 if tag == "d" else 1e-5), err

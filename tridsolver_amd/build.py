@@ -8,6 +8,7 @@ travels with the source tree; hipcc cross-compiles without a GPU present.
 Tools (tridsolver_amd/apps/bin/):
     adi_hip_d, adi_hip_s   ADI application (FP64 / FP32) on the library
     compare_d, compare_s   SumOfDiff comparison of two raw ADI outputs
+    blktrid_hip_d, _s      block-tridiagonal benchmark driver (FP64 / FP32)
 """
 from __future__ import annotations
 
@@ -91,6 +92,11 @@ def build_apps(verbose: bool = True) -> None:
                      "-Wl,-rpath,$ORIGIN/../..", "-o", os.path.join(BIN, f"adi_hip_{tag}")])
         jobs.append(["g++", "-O2", "-std=c++17", "-Wall", f"-DFPPREC={prec}",
                      os.path.join(APPS, "compare.cpp"), "-o", os.path.join(BIN, f"compare_{tag}")])
+        jobs.append([HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}",
+                     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                     "-Wno-unused-command-line-argument", f"-DFPPREC={prec}",
+                     os.path.join(APPS, "blktrid_hip.cpp"), "-L" + HERE, "-ltridhip",
+                     "-Wl,-rpath,$ORIGIN/../..", "-o", os.path.join(BIN, f"blktrid_hip_{tag}")])
     procs = [(j, subprocess.Popen(j, stdout=subprocess.PIPE, stderr=subprocess.STDOUT))
              for j in jobs]
     for j, p in procs:
