@@ -8,8 +8,13 @@
  * (A_0 and C_{N-1} are never read).  Layout as the reference GPU solver
  * (block/src/blktrid_gpu.cuh:265-431): block (n, p) is row-major at
  * (n*P + p)*M*M, vector entry (n, p) at (n*P + p)*M.  Cstar (N*P*M*M) and
- * dstar (N*P*M) are caller-provided scratch that receives the forward
- * sweep's C'_n = X_n^{-1} C_n and d'_n.  All pointers are device pointers.
+ * dstar (N*P*M) are caller-provided scratch that receives the elimination's
+ * intermediates: in the one-ended layouts the downward sweep's
+ * C'_n = X_n^{-1} C_n (n < N-1) and d'_n (every n); in the two-ended one
+ * (the default for most block sizes, see tridBlkForceLanes) C'_n and d'_n
+ * at rows 0 .. N/2-1 and the upward sweep's A''_n = X_n^{-1} A_n and d''_n
+ * at rows N/2+1 .. N-1.  The solution is u either way.  All pointers are
+ * device pointers.
  *
  * The reference declares these with C++ linkage and no status.  Here they
  * are declared extern "C"; the library exports them with C++ linkage too
@@ -76,7 +81,10 @@ EXTERN_C tridStatus_t tridDbtsvAdjoint(const double *A, const double *B,
                                        int blkdim, int sync);
 
 /* Forces the lanes-per-problem of the block kernels (0 = automatic; else a
- * divisor of blkdim) -- for tests. */
+ * divisor of blkdim; 100 + L, L = 1, 2 or 4 dividing blkdim, for the
+ * two-ended elimination with L lanes per half: one half eliminates
+ * downward to row N/2, the other upward, two dependency chains of N/2 rows
+ * per problem) -- for tests. */
 EXTERN_C tridStatus_t tridBlkForceLanes(int lanes);
 
 #endif

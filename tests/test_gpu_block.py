@@ -56,6 +56,48 @@ def test_block_every_lane_mapping(M, L):
     assert oracle.rel_err(exp, u.cpu().numpy()) <= 1e-12
 
 
+@pytest.mark.parametrize("M,L", [(1, 1), (2, 1), (2, 2), (3, 1), (4, 1), (4, 2), (4, 4), (5, 1),
+                                 (6, 2), (8, 2), (8, 4), (10, 2)])
+@pytest.mark.parametrize("N", [1, 2, 3, 8, 17])
+def test_block_two_ended_layouts(M, L, N):
+    """The two-ended elimination (force_lanes(100 + L): L lanes per half;
+    the upper half eliminates rows 0..N/2-1 downward, the lower half the rest
+    upward, both meet at row N/2), forward and transposed, against the
+    oracle; odd and even N, and N = 1, 2, 3 where a half is empty."""
+    import torch
+    P = 130
+    A, B, C, d = blk_system(M, N, P, seed=30 + M + N)
+    exp = oracle.btsv(A, B, C, d, N, P, M, layout=0)
+    t = [dev(x) for x in (A, B, C, d)]
+    blk.force_lanes(100 + L)
+    try:
+        u = blk.btsv(*t)
+        g = torch.ones_like(u)
+        lam = blk.btsv_adjoint(t[0], t[1], t[2], u, g)[3]
+    finally:
+        blk.force_lanes(0)
+    err = oracle.rel_err(exp, u.cpu().numpy())
+    assert err <= 1e-12, f"M={M} L={L} N={N}: {err:.2e}"
+    # the transposed solve through the dot identity <A^{-1} d, 1> = <d, A^{-T} 1>
+    lhs, rhs = float(u.sum()), float((t[3] * lam).sum())
+    assert abs(lhs - rhs) <= 1e-10 * max(1.0, abs(lhs)), (M, L, N, lhs, rhs)
+
+
+def test_block_force_lanes_rejects():
+    import tridsolver_amd as ta
+    for bad in (11, 100, 103, 105, -1):
+        with pytest.raises(ta.TridError):
+            blk.force_lanes(bad)
+    M, N, P = 3, 4, 10
+    A, B, C, d = blk_system(M, N, P, seed=1)
+    blk.force_lanes(102)  # 2 lanes per half do not divide M = 3
+    try:
+        with pytest.raises(ta.TridError):
+            blk.btsv(*(dev(x) for x in (A, B, C, d)))
+    finally:
+        blk.force_lanes(0)
+
+
 @pytest.mark.parametrize("M", [2, 3, 4, 8])
 def test_block_tuned_and_default_layouts(M):
     """With autotuning on, the first call of a shape times every layout (the

@@ -197,6 +197,222 @@ __device__ __forceinline__ void blk_problem(const BlkArgs<T> &g, long long p,
   }
 }
 
+// Two-ended ("twisted") form of blk_problem: each problem has two halves of
+// L lanes.  Half 0 eliminates rows 0 .. m-1 downward (C'_n, d'_n as in
+// blk_problem), half 1 rows N-1 .. m+1 upward -- the same recurrence on the
+// rows in reverse order with the sub- and super-diagonal blocks exchanged:
+// A''_n = X_n^{-1} A_n, d''_n = X_n^{-1}(d_n - C_n d''_{n+1}), X_n = B_n -
+// C_n A''_{n+1} -- so that every problem runs two dependency chains of N/2
+// block rows instead of one of N.  Both go to Cs/ds at their own rows (rows
+// past m hold A'', d'').  The middle row m (m = N/2) takes both neighbours:
+// (B_m - A_m C'_{m-1} - C_m A''_{m+1}) u_m = d_m - A_m d'_{m-1} - C_m
+// d''_{m+1}; every lane of both halves forms it with the same operations in
+// the same order (the partner half's C' or A'' columns and d' or d'' come
+// through lane reads), so both hold the same u_m.  The halves then
+// substitute outward: u_n = d'_n - C'_n u_{n+1} above m, u_n = d''_n - A''_n
+// u_{n-1} below.  h: the lane's half; gbase: its half's first lane; partner:
+// the lane of the other half that holds the same columns.
+template <typename T, int M, int L, int TRANS>
+__device__ __forceinline__ void blk_problem_tw(const BlkArgs<T> &g, long long p, bool active,
+                                               int h, int j, int gbase, int partner) {
+  constexpr int CM = M / L;
+  const long long P = g.P;
+  const int N = g.N;
+  const int m = N / 2;
+  const int cnt = h == 0 ? m : N - 1 - m;  // rows of this half
+  auto blk = [&](int n) { return ((long long)n * P + p) * (M * M); };
+  auto vec = [&](int n) { return ((long long)n * P + p) * M; };
+  auto sub = [&](int n, int i, int k) -> T {
+    return TRANS ? g.C[blk(n - 1) + k * M + i] : g.A[blk(n) + i * M + k];
+  };
+  auto diag = [&](int n, int i, int c) -> T {
+    return TRANS ? g.B[blk(n) + c * M + i] : g.B[blk(n) + i * M + c];
+  };
+  auto sup = [&](int n, int i, int c) -> T {
+    return TRANS ? g.A[blk(n + 1) + c * M + i] : g.C[blk(n) + i * M + c];
+  };
+  // row of step s of this half, and its couplings to the previous step's
+  // row (back) and toward the middle row (fwd)
+  auto rowof = [&](int s) { return h == 0 ? s : N - 1 - s; };
+  auto back = [&](int r, int i, int k) -> T { return h == 0 ? sub(r, i, k) : sup(r, i, k); };
+  auto fwd = [&](int r, int i, int c) -> T { return h == 0 ? sup(r, i, c) : sub(r, i, c); };
+
+  T Cp[M][CM], dp[M];
+#pragma unroll
+  for (int i = 0; i < M; i++) {
+    dp[i] = T(0);
+#pragma unroll
+    for (int t = 0; t < CM; t++) Cp[i][t] = T(0);
+  }
+  // [X | Cn | v] <- X^{-1} [X | Cn | v] (Gauss-Jordan without pivoting, as
+  // blk_problem; the lanes of one half share a block's columns)
+  auto gauss_jordan = [&](T (&X)[M][CM], T (&Cn)[M][CM], T (&v)[M]) {
+#pragma unroll
+    for (int q = 0; q < M; q++) {
+      T col[M];
+      const int tq = q / L;
+#pragma unroll
+      for (int i = 0; i < M; i++)
+        col[i] = (L == 1) ? X[i][tq] : lane_read(X[i][tq], min(gbase + (q % L), kWave - 1));
+      const T inv = rcp(col[q]);
+#pragma unroll
+      for (int t = 0; t < CM; t++) {
+        X[q][t] *= inv;
+        Cn[q][t] *= inv;
+      }
+      v[q] *= inv;
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+        if (i == q) continue;
+        const T f = col[i];
+#pragma unroll
+        for (int t = 0; t < CM; t++) {
+          X[i][t] -= f * X[q][t];
+          Cn[i][t] -= f * Cn[q][t];
+        }
+        v[i] -= f * v[q];
+      }
+    }
+  };
+  // X -= K Q, v -= K q for the row's coupling block K (entry (i, k) from kf)
+  // and the neighbour's eliminated Q (this lane's columns) and q
+  auto couple = [&](T (&X)[M][CM], T (&v)[M], auto kf, const T (&Q)[M][CM], const T (&q)[M]) {
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+      T kcol[M];
+#pragma unroll
+      for (int i = 0; i < M; i++) kcol[i] = kf(i, k);
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+#pragma unroll
+        for (int t = 0; t < CM; t++) X[i][t] -= kcol[i] * Q[k][t];
+        v[i] -= kcol[i] * q[k];
+      }
+      if constexpr (M >= 6) {
+        if ((k & 1) == 1) __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  };
+
+  for (int s = 0; s < cnt; s++) {
+    const int r = rowof(s);
+    T X[M][CM], Cn[M][CM], v[M];
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+#pragma unroll
+        for (int t = 0; t < CM; t++) {
+          const int c = j + L * t;
+          X[i][t] = diag(r, i, c);
+          Cn[i][t] = fwd(r, i, c);
+        }
+        v[i] = g.d[vec(r) + i];
+      }
+      if (s > 0) couple(X, v, [&](int i, int k) { return back(r, i, k); }, Cp, dp);
+    } else {
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+        v[i] = T(0);
+#pragma unroll
+        for (int t = 0; t < CM; t++) {
+          X[i][t] = (j + L * t == i) ? T(1) : T(0);
+          Cn[i][t] = T(0);
+        }
+      }
+    }
+    gauss_jordan(X, Cn, v);
+    if (active) {
+#pragma unroll
+      for (int t = 0; t < CM; t++) {
+        const int c = j + L * t;
+#pragma unroll
+        for (int i = 0; i < M; i++) g.Cs[blk(r) + i * M + c] = Cn[i][t];
+      }
+#pragma unroll
+      for (int i = 0; i < M; i++)
+        if (i % L == j) g.ds[vec(r) + i] = v[i];
+    }
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+      dp[i] = v[i];
+#pragma unroll
+      for (int t = 0; t < CM; t++) Cp[i][t] = Cn[i][t];
+    }
+  }
+
+  // the middle row: the upper neighbour's C', d' and the lower one's A'', d''
+  // in the order (upper, lower) on both halves
+  T Qo[M][CM], qo[M];
+#pragma unroll
+  for (int i = 0; i < M; i++) {
+    qo[i] = lane_read(dp[i], partner);
+#pragma unroll
+    for (int t = 0; t < CM; t++) Qo[i][t] = lane_read(Cp[i][t], partner);
+  }
+  T un[M];
+  {
+    T X[M][CM], Cn[M][CM], v[M];
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+#pragma unroll
+        for (int t = 0; t < CM; t++) {
+          X[i][t] = diag(m, i, j + L * t);
+          Cn[i][t] = T(0);
+        }
+        v[i] = g.d[vec(m) + i];
+      }
+      if (m > 0) {
+        if (h == 0) couple(X, v, [&](int i, int k) { return sub(m, i, k); }, Cp, dp);
+        else couple(X, v, [&](int i, int k) { return sub(m, i, k); }, Qo, qo);
+      }
+      if (m < N - 1) {
+        if (h == 0) couple(X, v, [&](int i, int k) { return sup(m, i, k); }, Qo, qo);
+        else couple(X, v, [&](int i, int k) { return sup(m, i, k); }, Cp, dp);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+        v[i] = T(0);
+#pragma unroll
+        for (int t = 0; t < CM; t++) {
+          X[i][t] = (j + L * t == i) ? T(1) : T(0);
+          Cn[i][t] = T(0);
+        }
+      }
+    }
+    gauss_jordan(X, Cn, v);
+#pragma unroll
+    for (int i = 0; i < M; i++) un[i] = v[i];
+  }
+  if (active && h == 0) {
+#pragma unroll
+    for (int i = 0; i < M; i++)
+      if (i % L == j) g.u[vec(m) + i] = un[i];
+  }
+  // substitution outward from the middle, each half over its own rows
+  for (int s = cnt - 1; s >= 0; s--) {
+    if (!active) continue;
+    const int r = rowof(s);
+    T nu[M];
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+      T acc = g.ds[vec(r) + i];
+#pragma unroll
+      for (int k = 0; k < M; k++) acc -= g.Cs[blk(r) + i * M + k] * un[k];
+      nu[i] = acc;
+      if constexpr (M >= 6) {
+        if ((i & 1) == 1) __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+      un[i] = nu[i];
+      if (i % L == j) g.u[vec(r) + i] = un[i];
+    }
+  }
+}
+
 // Prefetching form of blk_problem (used where blk_prefetch says): row n+1's
 // blocks are loaded
 // while row n is eliminated, and row n-1's C'/d' while row n is
@@ -420,6 +636,30 @@ __global__ void __launch_bounds__(256, blk_waves(M, L)) k_blk(BlkArgs<T> g) {
 template <int M, int L>
 constexpr long long blk_blocks(long long P) {
   return (P + 4 * (kWave / L) - 1) / (4 * (kWave / L));
+}
+
+// the two-ended form: 2L lanes per problem (half 0 = lanes 0..L-1 of the
+// group, half 1 = lanes L..2L-1), 64/(2L) problems per wave
+template <typename T, int M, int L, int TRANS>
+__global__ void __launch_bounds__(256, blk_min_waves(M)) k_blk_tw(BlkArgs<T> g) {
+  constexpr int PW = kWave / (2 * L);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x >> 6;
+  const int grp = lane / (2 * L);
+  const int in = lane - grp * 2 * L;
+  const int h = in / L, j = in - h * L;
+  const int gbase = grp * 2 * L + h * L;
+  const int partner = grp * 2 * L + (1 - h) * L + j;
+  const long long p = ((long long)blockIdx.x * 4 + wave) * PW + grp;
+  const bool active = grp < PW && p < g.P;
+  // idle lanes past the last whole group read from themselves
+  blk_problem_tw<T, M, L, TRANS>(g, p, active, h, j, min(gbase, kWave - 1),
+                                 grp < PW ? partner : lane);
+}
+
+template <int M, int L>
+constexpr long long blk_tw_blocks(long long P) {
+  return (P + 4 * (kWave / (2 * L)) - 1) / (4 * (kWave / (2 * L)));
 }
 
 // ---- one lane per block column, tiles through LDS ---------------------------
@@ -724,8 +964,10 @@ namespace {
 std::atomic<int> g_force_lanes{0};
 
 // layout: kTiled = the tiled kernel (k_blk2, M = 3..6), L > 0 = k_blk with
-// L lanes per problem, 0 = the mixed launch's layout for M
+// L lanes per problem, kTwist + L = k_blk_tw with L lanes per half, 0 = the
+// mixed launch's layout for M
 constexpr int kTiled = -1;
+constexpr int kTwist = 100;
 
 template <typename T, int M, int TRANS>
 hipError_t launch_m(const BlkArgs<T> &g, int lanes, hipStream_t st) {
@@ -741,6 +983,17 @@ hipError_t launch_m(const BlkArgs<T> &g, int lanes, hipStream_t st) {
       return hipGetLastError();
     }
   }
+#define TCASE(LL)                                                                \
+  if constexpr (M % LL == 0)                                                     \
+    if (lanes == kTwist + LL) {                                                  \
+      hipLaunchKernelGGL((k_blk_tw<T, M, LL, TRANS>),                            \
+                         dim3((unsigned)blk_tw_blocks<M, LL>(g.P)), dim3(256), 0, st, g); \
+      return hipGetLastError();                                                  \
+    }
+  TCASE(1)
+  TCASE(2)
+  TCASE(4)
+#undef TCASE
 #define LCASE(LL)                                                                \
   if constexpr (M % LL == 0)                                                     \
     if (L == LL) {                                                               \
@@ -786,22 +1039,32 @@ tridStatus_t check_blk(const void *A, const void *B, const void *C, const void *
   if (N > 0 && P > 0 && (!B || !d || !u || (N > 1 && (!A || !C))))
     return fail(TRID_STATUS_INVALID_VALUE, "NULL array");
   const int L = g_force_lanes.load();
-  if (L && M % L != 0) return fail(TRID_STATUS_INVALID_VALUE, "forced lanes do not divide blkdim");
+  const int Lh = L > kTwist ? L - kTwist : L;  // lanes per half of the two-ended form
+  if (L && M % Lh != 0) return fail(TRID_STATUS_INVALID_VALUE, "forced lanes do not divide blkdim");
   return TRID_STATUS_SUCCESS;
 }
 
-// Layout of a batch without tuning, from sweeps on MI355X at N = 128,
-// P = 65,536 (ms, FP64): one lane per problem wins for M <= 3 once the
-// batch fills the chip with it (M = 2: 0.350 against 0.369 with two lanes;
-// M = 3: 0.919 against 0.989 tiled, 1.058 with three lanes), four lanes for
-// M = 8 (5.08 against 5.72 with eight, 6.36 with two); the tiled kernel for
-// M = 4, 5 (1.34 against 1.46 / 1.63 / 2.13 with 4 / 2 / 1 lanes; 2.26
-// against 2.67 / 3.76).  Smaller batches keep more lanes per problem.
-int default_layout(int M, long long P) {
-  if (P >= 65536) {
-    if (M <= 3) return 1;
-    if (M == 8) return 4;
-  }
+// Layout of a batch without tuning, from the tuner's sweeps on MI355X at
+// N = 128 over P = 4096 / 16384 / 65536 (one column each below: P < 8192,
+// P < 65536, larger), M = 2..6, 8, FP64 and FP32.  The two-ended form wins
+// most of them (FP64, ms of the tuning runs: M = 2, P = 4096: 0.201 against
+// 0.370 with one lane; M = 8, P = 16384: 3.73 against 4.08 with four lanes;
+// M = 3, P = 65536: 2.33 against 2.59; FP32 M = 5, P = 65536: 3.01 against
+// 4.02 tiled); the tiled kernel keeps the mid sizes of FP64 M = 3..5 (M = 4,
+// P = 16384: 0.964 against 1.054 two-ended) and one lane FP64 M = 2 at
+// P = 65536 (1.038 against 1.063).  0: not swept, the tiled kernel where it
+// exists, else default_lanes.
+int default_layout(int M, long long P, int ts) {
+  constexpr int T = kTiled, W1 = kTwist + 1, W2 = kTwist + 2, W4 = kTwist + 4;
+  static constexpr int k64[9][3] = {{0, 0, 0},   {0, 0, 0},    {W1, W1, 1}, {T, T, W1},
+                                    {W4, T, T},  {T, T, W1},   {T, W2, W2}, {0, 0, 0},
+                                    {W4, W4, W4}};
+  static constexpr int k32[9][3] = {{0, 0, 0},   {0, 0, 0},     {W2, W2, W1}, {W1, W1, W1},
+                                    {W4, W4, W1}, {W1, W1, W1}, {T, W2, W1},  {0, 0, 0},
+                                    {W4, W4, W1}};
+  const int band = P < 8192 ? 0 : (P < 65536 ? 1 : 2);
+  const int t = M <= 8 ? (ts == 8 ? k64 : k32)[M][band] : 0;
+  if (t) return t;
   return blk2_sizes(M) ? kTiled : default_lanes(M);
 }
 
@@ -818,7 +1081,7 @@ template <typename T, int TRANS>
 int choose_layout(const BlkArgs<T> &g, int M, hipStream_t st) {
   const int forced = g_force_lanes.load();
   if (forced) return forced;
-  const int def = default_layout(M, g.P);
+  const int def = default_layout(M, g.P, (int)sizeof(T));
   if (autotune_mode() != 1) return def;
   const auto key = std::make_tuple((int)sizeof(T), TRANS, M, g.N, g.P);
   {
@@ -834,6 +1097,9 @@ int choose_layout(const BlkArgs<T> &g, int M, hipStream_t st) {
   if (blk2_sizes(M) && def != kTiled) cands.push_back(kTiled);
   for (int L = 1; L <= M && L <= 8; L++)
     if (M % L == 0 && L != def) cands.push_back(L);
+  // the two-ended form with 1, 2 or 4 lanes per half
+  for (int L : {1, 2, 4})
+    if (M % L == 0 && kTwist + L != def) cands.push_back(kTwist + L);
   std::vector<float> tm;
   auto run = [&](int i) { return launch_blk<T, TRANS>(g, M, cands[i], st); };
   const int i = pick_fastest((int)cands.size(), [](int) { return true; }, run, st, &tm, 3, 2);
@@ -921,9 +1187,11 @@ EXTERN_C tridStatus_t tridDbtsvMixedBatch(const tridDbtsvGroup_t *groups,
     a.g[k] = BlkArgs<double>{G.A, G.B, G.C, G.d, G.Cstar, G.dstar, G.u, G.N, G.P};
     a.M[k] = G.M;
     a.first_block[k] = nb;
-    // the untuned layout of a separate launch where the mixed kernel has it
-    const int dl = default_layout(G.M, G.P);
-    const int L = (dl == 1 && G.M <= 3) || (dl == 4 && G.M == 8) ? dl : default_lanes(G.M);
+    // one lane per problem for M <= 3 and four for M = 8 in batches that
+    // fill the chip with them (the one-ended layouts that beat M lanes at
+    // P = 65536), else default_lanes
+    const bool big = G.P >= 65536;
+    const int L = big && G.M <= 3 ? 1 : (big && G.M == 8 ? 4 : default_lanes(G.M));
     a.L[k] = L;
     const long long per = 4LL * (kWave / L);
     nb += (G.P + per - 1) / per;
@@ -979,7 +1247,9 @@ EXTERN_C tridStatus_t tridDbtsvAdjoint(const double *A, const double *B,
 }
 
 EXTERN_C tridStatus_t tridBlkForceLanes(int lanes) {
-  if (lanes < 0 || lanes > 10) return fail(TRID_STATUS_INVALID_VALUE, "lanes must be in [0, 10]");
+  if ((lanes < 0 || lanes > 10) && lanes != kTwist + 1 && lanes != kTwist + 2 &&
+      lanes != kTwist + 4)
+    return fail(TRID_STATUS_INVALID_VALUE, "lanes must be in [0, 10] or 101, 102, 104");
   g_force_lanes.store(lanes);
   return TRID_STATUS_SUCCESS;
 }
