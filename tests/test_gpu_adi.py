@@ -154,14 +154,16 @@ def test_adi_every_sweep_layout(shape, dt):
     print(f"{shape} {dt.__name__}: {ran} layouts")
 
 
-@pytest.mark.parametrize("shape", [(64, 40, 24), (256, 24, 30), (32, 8, 5)])
+@pytest.mark.parametrize("shape", [(64, 40, 24), (256, 24, 30), (32, 8, 5), (128, 16, 6), (512, 6, 5)])
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 def test_adi_every_fused_x_layout(shape, dt):
     """The fused right-hand side + x stage pinned (tridAdiPlanSetSweep dir 0)
     to every chunk length, both block orders (the XCD-contiguous order
     included) and both forms of the x sweep (precomputed factors, pair = 1,
     or factors computed in the kernel), against the oracle; rows that do not
-    fit are rejected."""
+    fit are rejected.  nx = 128, 256 and 512 put one, two or four 64-lane
+    rows of 16-byte pieces in a line, the sizes at which a wave with two or
+    more lines takes its y-neighbour lines from its own span."""
     import torch
     nx, ny, nz = shape
     u0 = rand_init(nx, ny, nz, dt, seed=nx + ny)

@@ -162,7 +162,11 @@ __device__ __forceinline__ void store_staged_pad(T *__restrict__ span, T *__rest
 // copies the table into LDS transposed (factor i of chunk k at i * cps + k),
 // so a wave's factor reads are consecutive words.  CP: cache policy of the
 // du stores.
-template <typename T, int R, int CP, int FR>
+// YS > 0 (nx = 64 * YS elements per 16-byte piece, so that a line is YS
+// piece rows of the span and the wave holds NP / YS >= 2 lines): the y-1 and
+// y+1 spans overlap the wave's own span by all but one line, so only their
+// outer line is loaded and the rest is taken from the wave's registers.
+template <typename T, int R, int CP, int FR, int YS = 0>
 __global__ void __launch_bounds__(256)
     k_adi_rhs_x(const T *__restrict__ u, T *__restrict__ du, int nx, int ny,
                 int nz, T lam, CoefConst<T> cf, int cps_log2, int xcd,
@@ -192,36 +196,54 @@ __global__ void __launch_bounds__(256)
       const long long nb8 = nblk & ~7LL;
       if (b0 < nb8) blk = (b0 & 7) * (nb8 >> 3) + (b0 >> 3);
     }
-    const long long line = blk * lines_per_block + (tid >> cps_log2);
-    const long long wline0 = blk * lines_per_block + ((tid & ~(kWave - 1)) >> cps_log2);
+    // the wave's first line, its plane and its place in the plane are
+    // wave-uniform (scalar registers; the host keeps ny * nz below 2^31); a
+    // wave never straddles planes
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const long long wline0 = blk * lines_per_block + ((wv * kWave) >> cps_log2);
     const bool wave_active = wline0 < nlines;
     const long long nxy = (long long)nx * ny;
-    const int kw = (int)(wline0 / ny);
+    const unsigned wl = (unsigned)wline0;
+    const int kw = (int)(wl / (unsigned)ny);
+    const int jw = (int)(wl - (unsigned)kw * (unsigned)ny);  // the wave's first line in its plane
     const bool plane_in = kw >= 1 && kw <= nz - 2;
-    const int jl = (int)(line % ny);
-    const bool line_in = plane_in && jl >= 1 && jl <= ny - 2;
     T *stage = reinterpret_cast<T *>(smem_raw) + (tid >> 6) * SW;
     const long long wo = wline0 * nx;
-    const int row0 = k * R;
 
     T d[R];
 #pragma unroll
     for (int r = 0; r < R; r++) d[r] = T(0);
     if (wave_active && plane_in) {
+      constexpr int E = Vec16<T>::n, NP = R / E;
       // all five spans in flight before the first transpose
       Span<T, R> sc, sym, syp, szm, szp;
       span_issue<T, R>(u + wo, lane, sc);
-      span_issue<T, R>(u + wo - nx, lane, sym);
-      span_issue<T, R>(u + wo + nx, lane, syp);
+      if constexpr (YS > 0 && YS < NP) {
+        // element e of the y-1 span is element e - nx = e - 64*E*YS of the
+        // wave's span: piece row v of it is row v - YS of sc (v >= YS); the
+        // y+1 span's row v is row v + YS of sc (v < NP - YS)
+        using V = typename Vec16<T>::type;
+        const V *gm = reinterpret_cast<const V *>(u + wo - nx);
+        const V *gp = reinterpret_cast<const V *>(u + wo + nx);
+#pragma unroll
+        for (int v = 0; v < YS; v++) sym.v[v] = gm[v * kWave + lane];
+#pragma unroll
+        for (int v = NP - YS; v < NP; v++) syp.v[v] = gp[v * kWave + lane];
+#pragma unroll
+        for (int v = YS; v < NP; v++) sym.v[v] = sc.v[v - YS];
+#pragma unroll
+        for (int v = 0; v < NP - YS; v++) syp.v[v] = sc.v[v + YS];
+      } else {
+        span_issue<T, R>(u + wo - nx, lane, sym);
+        span_issue<T, R>(u + wo + nx, lane, syp);
+      }
       span_issue<T, R>(u + wo - nxy, lane, szm);
       span_issue<T, R>(u + wo + nxy, lane, szp);
       // the stencil is evaluated in the loaded layout (lane holds elements
       // (v*64 + lane)*E .. +E-1 of the wave's span): the neighbour spans line
       // up element for element, the x-neighbours at piece ends come from the
       // adjacent lanes; only the right-hand side is transposed into chunks
-      constexpr int E = Vec16<T>::n, NP = R / E;
       const int nx_log2 = __builtin_ctz(nx);  // nx = cps * R, a power of two
-      const int jw = (int)(wline0 % ny);      // the wave's first line in its plane
       Span<T, R> sd;
 #pragma unroll
       for (int v = 0; v < NP; v++) {
@@ -284,14 +306,31 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int i = 1; i < R - 1; i++) y[i] = d[i] - fv(Ly::EA + i - 1) * xf - fv(Ly::EC + i - 1) * rd;
     } else {
+      // the interior line's matrix: constant rows, identity rows at the
+      // grid's first row (row 0 of chunk 0) and last row (row R-1 of chunk
+      // cps-1).  Lines off the interior have d = 0, so this matrix gives them
+      // their zero solution too (as the factor form above).  The empty asm
+      // ties the coefficients to the staged right-hand side: hoisted above
+      // the span loads, the factor arithmetic would hold its results across
+      // their latency (163 against 104 VGPRs at R = 8, FP64).
+      T ca = cf.a, cb = cf.b, cc = cf.c;
+      asm volatile("" : "+v"(ca), "+v"(cb), "+v"(cc) : "v"(d[0]));
       T a[R], b[R], c[R];
 #pragma unroll
       for (int r = 0; r < R; r++) {
-        const int row = row0 + r;
-        const bool in = line_in && row >= 1 && row + 2 <= nx;
-        a[r] = in ? cf.a : T(0);
-        b[r] = in ? cf.b : T(1);
-        c[r] = in ? cf.c : T(0);
+        a[r] = ca;
+        b[r] = cb;
+        c[r] = cc;
+      }
+      if (k == 0) {
+        a[0] = T(0);
+        b[0] = T(1);
+        c[0] = T(0);
+      }
+      if (k == cps - 1) {
+        a[R - 1] = T(0);
+        b[R - 1] = T(1);
+        c[R - 1] = T(0);
       }
       Row<T> f, l;
       chunk_reduce<T, R>(a, b, c, d, f, l);
@@ -395,13 +434,29 @@ hipError_t launch_fused(tridAdiPlan_st *p, T *u, T *du, hipStream_t st) {
   }
   // the factor-table form runs persistent: as many work-groups as are
   // resident at once, each copying the table into LDS once
-#define TRID_FUSED(RR, CC, FF)                                                                \
+  // piece rows per line when a line is a whole number (< NP) of them
+  const int E = Vec16<T>::n;
+  const int ys = p->nx % (kWave * E) == 0 ? p->nx / (kWave * E) : 0;
+#define TRID_FUSED_K(RR, CC, FF, YY)                                                          \
   do {                                                                                        \
-    auto kern = k_adi_rhs_x<T, RR, CC, FF>;                                                   \
+    auto kern = k_adi_rhs_x<T, RR, CC, FF, YY>;                                               \
     unsigned g = blocks;                                                                      \
     if (FF) g = std::min(g, resident_blocks(reinterpret_cast<const void *>(kern), 256, lds)); \
     (void)launch_kernel(kern, dim3(g), dim3(256), (std::uint32_t)lds, st, u, du, p->nx,      \
                         p->ny, p->nz, (T)p->lambda, cf, p->fused_cps_log2, p->fused_xcd, F); \
+  } while (0)
+  // the shared-y-span forms (YS = 1, 2, 4 where YS < R / E); everything
+  // else loads all five spans
+#define TRID_FUSED(RR, CC, FF)                                             \
+  do {                                                                     \
+    constexpr int NP_ = RR / Vec16<T>::n;                                  \
+    if constexpr (NP_ > 1)                                                 \
+      if (ys == 1) { TRID_FUSED_K(RR, CC, FF, 1); break; }                 \
+    if constexpr (NP_ > 2)                                                 \
+      if (ys == 2) { TRID_FUSED_K(RR, CC, FF, (NP_ > 2 ? 2 : 0)); break; } \
+    if constexpr (NP_ > 4)                                                 \
+      if (ys == 4) { TRID_FUSED_K(RR, CC, FF, (NP_ > 4 ? 4 : 0)); break; } \
+    TRID_FUSED_K(RR, CC, FF, 0);                                           \
   } while (0)
 #define TRID_FUSED_R(CC, FF)                        \
   if (p->fused_R == 8) TRID_FUSED(8, CC, FF);       \
@@ -422,6 +477,7 @@ hipError_t launch_fused(tridAdiPlan_st *p, T *u, T *du, hipStream_t st) {
   }
 #undef TRID_FUSED_R
 #undef TRID_FUSED
+#undef TRID_FUSED_K
   return hipGetLastError();
 }
 
@@ -634,10 +690,12 @@ tridStatus_t steps_any(tridAdiPlan_st *p, void *u, void *du, int nsteps, hipStre
   return TRID_STATUS_SUCCESS;
 }
 
-// The fused kernel needs nx == cps * R with cps <= 64 a power of two and ny
-// a multiple of the lines per wave (64 / cps).
+// The fused kernel needs nx == cps * R with cps <= 64 a power of two, ny a
+// multiple of the lines per wave (64 / cps) and fewer than 2^31 lines (its
+// line indices are 32-bit).
 bool fused_rows_ok(const tridAdiPlan_st *p, int R) {
   if (R != 4 && R != 8 && R != 16) return false;
+  if ((long long)p->ny * p->nz >= (1LL << 31)) return false;
   if (p->nx % R) return false;
   const int cps = p->nx / R;
   if (cps > kWave || (cps & (cps - 1))) return false;
