@@ -271,8 +271,11 @@ __global__ void __launch_bounds__(256)
     }
     T y[R];
     if constexpr (FR) {
-      // factor i of this lane's chunk
-      auto fv = [&](int i) { return Ft[i * cps + k]; };
+      // factor i of this lane's chunk (the empty asm keeps the table reads
+      // after the staged right-hand side, as for the coefficients below)
+      int kf = k;
+      asm volatile("" : "+v"(kf) : "v"(d[0]));
+      auto fv = [&](int i) { return Ft[i * cps + kf]; };
       // chunk sweep (d only), the recurrences of k_cc_sweep
       T dp = d[1] * fv(Ly::R1);
       d[1] = dp;
