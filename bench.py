@@ -85,9 +85,12 @@ def parse():
     return ap.parse_args()
 
 
-def kernel_of_config(cfg: str, dtype: str, inc: int, coef: str):
+def kernel_of_config(cfg: str, dtype: str, inc: int, coef: str, nx: int = 0):
     """The kernel (template instance) that runs a configuration string of the
-    library (tridGetLastConfig, an ADI plan's stage), as rocprofv3 names it."""
+    library (tridGetLastConfig, an ADI plan's stage), as rocprofv3 names it.
+    nx: the grid's x extent, for the fused right-hand side + x kernel (its
+    last template argument, YS, is the number of 64-lane rows of 16-byte
+    pieces in an x-line when a wave holds two or more lines, else 0)."""
     import re
     T = "double" if dtype == "f64" else "float"
     # the last template argument is the cache policy (0 non-temporal, 1 ordinary,
@@ -109,7 +112,10 @@ def kernel_of_config(cfg: str, dtype: str, inc: int, coef: str):
     m = re.search(r"rhs\+x\(fused,R(\d+)((?:,\w+)*)\)", cfg)
     if m:
         fl = m.group(2).split(",")
-        return f"k_adi_rhs_x<{T}, {m.group(1)}, {int('cached' in fl)}, {int('cc' in fl)}>"
+        R, E = int(m.group(1)), (2 if T == "double" else 4)
+        ys = nx // (64 * E) if nx and nx % (64 * E) == 0 else 0
+        ys = ys if ys in (1, 2, 4) and ys < R // E else 0
+        return f"k_adi_rhs_x<{T}, {R}, {int('cached' in fl)}, {int('cc' in fl)}, {ys}>"
     return None
 
 
@@ -953,7 +959,7 @@ def main():
     if "plan" in W:
         out["config"]["plan"] = W["plan"]
     kname = kernel_of_config(dom_cfg, args.dtype, 1 if W["kernels"][dom] == "sweep_z_inc" else 0,
-                             "CoefConst" if args.workload == "adi" else "CoefArrays")
+                             "CoefConst" if args.workload == "adi" else "CoefArrays", args.n)
     out["roofline"]["kernel_name"] = kname
     out["roofline"]["traffic"], out["roofline"]["traffic_source"] = traffic_from_profiles(
         f"{args.workload}{args.n}|{args.dtype}|{kname}")
