@@ -101,10 +101,11 @@ def test_block_force_lanes_rejects():
 @pytest.mark.parametrize("M", [2, 3, 4, 8])
 def test_block_tuned_and_default_layouts(M):
     """With autotuning on, the first call of a shape times every layout (the
-    tiled kernel and each lane count that divides M) and keeps the fastest;
-    the untuned default takes one lane per problem for M <= 3 and four for
-    M = 8 on batches of 65,536 problems or more.  Every layout solves as the
-    oracle does, forward and transposed (adjoint)."""
+    tiled kernel, each lane count that divides M and the two-ended forms) and
+    keeps the fastest; the untuned default comes from a table per precision,
+    block size and batch band (two-ended for most FP64 entries at these
+    sizes).  Every layout solves as the oracle does, forward and transposed
+    (adjoint)."""
     import tridsolver_amd as ta
     import torch
     for N, P in ((6, 3000), (3, 65536)):
