@@ -958,13 +958,31 @@ __device__ __forceinline__ void part_block(const Coef &coef, T *__restrict__ Dd,
       }
     }
   } else if constexpr (!Coef::kArrays) {
+    // the line's rows (identity rows on lines off the interior), then
+    // identity at row 0 and at rows N-1 and past: one select per
+    // coefficient and lane, and per row only in the chunks that hold row 0
+    // or row N-1
+    const T ca = line_in ? coef.a : T(0), cb = line_in ? coef.b : T(1),
+            cc = line_in ? coef.c : T(0);
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      const int row = row0 + r;
-      const bool in = line_in && row >= 1 && row + 2 <= N;
-      a[r] = in ? coef.a : T(0);
-      b[r] = in ? coef.b : T(1);
-      c[r] = in ? coef.c : T(0);
+      a[r] = ca;
+      b[r] = cb;
+      c[r] = cc;
+    }
+    if (row0 == 0) {
+      a[0] = T(0);
+      b[0] = T(1);
+      c[0] = T(0);
+    }
+    if (row0 + R >= N) {
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (row0 + r + 1 >= N) {
+          a[r] = T(0);
+          b[r] = T(1);
+          c[r] = T(0);
+        }
     }
   }
   if (GSTAGE) {
