@@ -84,7 +84,8 @@ EXTERN_C tridStatus_t tridDbtsvAdjoint(const double *A, const double *B,
  * divisor of blkdim; 100 + L, L = 1, 2 or 4 dividing blkdim, for the
  * two-ended elimination with L lanes per half: one half eliminates
  * downward to row N/2, the other upward, two dependency chains of N/2 rows
- * per problem) -- for tests. */
+ * per problem; 200 for the two-ended elimination on the tiled kernel, M
+ * lanes per half, blkdim 3..6) -- for tests. */
 EXTERN_C tridStatus_t tridBlkForceLanes(int lanes);
 
 #endif

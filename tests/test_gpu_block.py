@@ -83,14 +83,50 @@ def test_block_two_ended_layouts(M, L, N):
     assert abs(lhs - rhs) <= 1e-10 * max(1.0, abs(lhs)), (M, L, N, lhs, rhs)
 
 
+@pytest.mark.parametrize("M", [3, 4, 5, 6])
+@pytest.mark.parametrize("N", [1, 2, 3, 8, 17])
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_block_two_ended_tiled(M, N, dt):
+    """The two-ended elimination on the tiled kernel (force_lanes(200): two
+    groups of M lanes per problem, each with its own LDS tile), forward and
+    transposed, against the oracle; P = 131 leaves a partial last wave."""
+    import torch
+    P = 131
+    A, B, C, d = blk_system(M, N, P, seed=50 + M + N)
+    exp = oracle.btsv(A, B, C, d, N, P, M, layout=0)
+    t = [dev(x.astype(dt)) for x in (A, B, C, d)]
+    blk.force_lanes(200)
+    try:
+        u = blk.btsv(*t)
+        lam = None
+        if dt == np.float64:
+            lam = blk.btsv_adjoint(t[0], t[1], t[2], u, torch.ones_like(u))[3]
+    finally:
+        blk.force_lanes(0)
+    err = oracle.rel_err(exp, u.cpu().numpy())
+    assert err <= (1e-12 if dt == np.float64 else 1e-5), f"M={M} N={N}: {err:.2e}"
+    if lam is not None:
+        # the transposed solve through <A^{-1} d, 1> = <d, A^{-T} 1>
+        lhs, rhs = float(u.sum()), float((t[3] * lam).sum())
+        assert abs(lhs - rhs) <= 1e-10 * max(1.0, abs(lhs)), (M, N, lhs, rhs)
+
+
 def test_block_force_lanes_rejects():
     import tridsolver_amd as ta
-    for bad in (11, 100, 103, 105, -1):
+    for bad in (11, 100, 103, 105, -1, 199, 201):
         with pytest.raises(ta.TridError):
             blk.force_lanes(bad)
     M, N, P = 3, 4, 10
     A, B, C, d = blk_system(M, N, P, seed=1)
     blk.force_lanes(102)  # 2 lanes per half do not divide M = 3
+    try:
+        with pytest.raises(ta.TridError):
+            blk.btsv(*(dev(x) for x in (A, B, C, d)))
+    finally:
+        blk.force_lanes(0)
+    M = 2
+    A, B, C, d = blk_system(M, N, P, seed=2)
+    blk.force_lanes(200)  # the tiled kernels take M = 3..6
     try:
         with pytest.raises(ta.TridError):
             blk.btsv(*(dev(x) for x in (A, B, C, d)))

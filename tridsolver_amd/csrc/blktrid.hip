@@ -872,9 +872,252 @@ __global__ void __launch_bounds__(256, blk2_min_waves(M)) k_blk2(BlkArgs<T> g) {
 // tiled kernel's register demand does not fit).
 __host__ __device__ constexpr bool blk2_sizes(int M) { return M >= 3 && M <= 6; }
 
+// Two-ended form of blk2_problem (the elimination of blk_problem_tw with the
+// tiled kernel's cooperative block moves): two groups of M lanes per
+// problem, each with its own tile.  Half 0 eliminates rows 0 .. m-1
+// downward (C'_n, d'_n), half 1 rows N-1 .. m+1 upward (A''_n, d''_n, the
+// sub- and super-diagonal blocks exchanging roles), m = N/2; both halves
+// form the middle row's system with the same operations in the same order
+// (the other half's eliminated columns through lane reads, partner = the
+// lane of the other half with the same column j) and substitute outward
+// through their tiles.  Tile roles: K = the coupling to the previous row of
+// the half (sub-diagonal for half 0, super-diagonal for half 1; sub(m) for
+// the middle row), F = the coupling toward the middle (super / sub; sup(m)).
+template <typename T, int M, int TRANS>
+__device__ __forceinline__ void blk2_problem_tw(const BlkArgs<T> &g, long long p, bool store,
+                                                bool lane_ok, int h, int j, int gbase,
+                                                int partner, T *tile) {
+  const long long P = g.P;
+  const int N = g.N;
+  const int m = N / 2;
+  const int cnt = h == 0 ? m : N - 1 - m;  // rows of this half
+  auto blk = [&](int n) { return ((long long)n * P + p) * (M * M); };
+  auto vec = [&](int n) { return ((long long)n * P + p) * M; };
+  T *sK = tile, *sB = tile + M * M, *sF = tile + 2 * M * M, *sd = tile + 3 * M * M;
+  T *su = sd + M;
+  auto wb = []() { __builtin_amdgcn_wave_barrier(); };
+  auto rowof = [&](int s) { return h == 0 ? s : N - 1 - s; };
+  // element e of the raw sub- and super-diagonal blocks of row n (C_{n-1}
+  // and A_{n+1} read transposed through at_t in TRANS mode)
+  auto ld_sub = [&](int n, int e) -> T { return TRANS ? g.C[blk(n - 1) + e] : g.A[blk(n) + e]; };
+  auto ld_sup = [&](int n, int e) -> T { return TRANS ? g.A[blk(n + 1) + e] : g.C[blk(n) + e]; };
+  auto at_t = [&](const T *t, int i, int c) { return TRANS ? t[c * M + i] : t[i * M + c]; };
+
+  T Cp[M], dp[M];
+#pragma unroll
+  for (int i = 0; i < M; i++) {
+    Cp[i] = T(0);
+    dp[i] = T(0);
+  }
+  // this lane's elements j, j+M, ... of the next row's K, B, F blocks and
+  // entry j of its d (the next row is loaded while this one is eliminated)
+  T rk[M], rb[M], rf[M], rd;
+  auto load_step = [&](int s) {
+    const int n = rowof(s);
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+      const int e = j + M * r;
+      rk[r] = s > 0 ? (h == 0 ? ld_sub(n, e) : ld_sup(n, e)) : T(0);
+      rb[r] = g.B[blk(n) + e];
+      rf[r] = h == 0 ? ld_sup(n, e) : ld_sub(n, e);
+    }
+    rd = g.d[vec(n) + j];
+  };
+  auto load_mid = [&]() {
+#pragma unroll
+    for (int r = 0; r < M; r++) {
+      const int e = j + M * r;
+      rk[r] = m > 0 ? ld_sub(m, e) : T(0);
+      rb[r] = g.B[blk(m) + e];
+      rf[r] = m < N - 1 ? ld_sup(m, e) : T(0);
+    }
+    rd = g.d[vec(m) + j];
+  };
+  auto put_tiles = [&]() {
+    if (lane_ok) {
+#pragma unroll
+      for (int r = 0; r < M; r++) {
+        const int e = j + M * r;
+        sK[e] = rk[r];
+        sB[e] = rb[r];
+        sF[e] = rf[r];
+      }
+      sd[j] = rd;
+    }
+    wb();
+  };
+  // X -= K Q, v -= K q for the tile t's block K and the eliminated columns
+  // Q (this lane's column) and vector q
+  auto couple = [&](T (&X)[M], T (&v)[M], const T *t, const T (&Q)[M], const T (&q)[M]) {
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+        const T kik = at_t(t, i, k);
+        X[i] -= kik * Q[k];
+        v[i] -= kik * q[k];
+      }
+      if constexpr (M >= 5) {
+        if ((k & 1) == 1) __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  };
+  // [X | Cn | v] <- X^{-1} [X | Cn | v], pivot columns from the half's lanes
+  auto gauss_jordan = [&](T (&X)[M], T (&Cn)[M], T (&v)[M]) {
+#pragma unroll
+    for (int q = 0; q < M; q++) {
+      T col[M];
+#pragma unroll
+      for (int i = 0; i < M; i++) col[i] = lane_read(X[i], min(gbase + q, kWave - 1));
+      const T inv = rcp(col[q]);
+      X[q] *= inv;
+      Cn[q] *= inv;
+      v[q] *= inv;
+#pragma unroll
+      for (int i = 0; i < M; i++) {
+        if (i == q) continue;
+        const T f = col[i];
+        X[i] -= f * X[q];
+        Cn[i] -= f * Cn[q];
+        v[i] -= f * v[q];
+      }
+    }
+  };
+
+  if (cnt > 0) load_step(0);
+  else load_mid();
+  for (int s = 0; s < cnt; s++) {
+    const int n = rowof(s);
+    put_tiles();
+    if (s + 1 < cnt) load_step(s + 1);
+    else load_mid();
+    T X[M], Cn[M], v[M];
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+      X[i] = at_t(sB, i, j);
+      Cn[i] = at_t(sF, i, j);
+      v[i] = sd[i];
+    }
+    if (s > 0) couple(X, v, sK, Cp, dp);
+    wb();
+    __builtin_amdgcn_sched_barrier(0);
+    gauss_jordan(X, Cn, v);
+    __builtin_amdgcn_sched_barrier(0);
+    // the eliminated F block (column j in this lane) and d to the scratch
+    if (lane_ok) {
+#pragma unroll
+      for (int i = 0; i < M; i++) sF[i * M + j] = Cn[i];
+    }
+    wb();
+    if (store) {
+#pragma unroll
+      for (int r = 0; r < M; r++) g.Cs[blk(n) + j + M * r] = sF[j + M * r];
+      g.ds[vec(n) + j] = v[j];
+    }
+    wb();
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+      Cp[i] = Cn[i];
+      dp[i] = v[i];
+    }
+  }
+
+  // the middle row: the upper neighbour's C', d' and the lower one's A'',
+  // d'' in the order (upper, lower) on both halves
+  T Qo[M], qo[M];
+#pragma unroll
+  for (int i = 0; i < M; i++) {
+    Qo[i] = lane_read(Cp[i], partner);
+    qo[i] = lane_read(dp[i], partner);
+  }
+  put_tiles();
+  T un[M];
+  {
+    T X[M], Cn[M], v[M];
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+      X[i] = at_t(sB, i, j);
+      Cn[i] = T(0);
+      v[i] = sd[i];
+    }
+    if (m > 0) {
+      if (h == 0) couple(X, v, sK, Cp, dp);
+      else couple(X, v, sK, Qo, qo);
+    }
+    if (m < N - 1) {
+      if (h == 0) couple(X, v, sF, Qo, qo);
+      else couple(X, v, sF, Cp, dp);
+    }
+    wb();
+    __builtin_amdgcn_sched_barrier(0);
+    gauss_jordan(X, Cn, v);
+#pragma unroll
+    for (int i = 0; i < M; i++) un[i] = v[i];
+  }
+  if (store && h == 0) g.u[vec(m) + j] = un[j];
+
+  // substitution outward from the middle over the half's rows: lane i forms
+  // u_n[i] = d*_n[i] - F*_n[i, :] u_prev from the row's tile
+  T bc[M], bd = T(0);
+  auto load_back = [&](int n) {
+#pragma unroll
+    for (int r = 0; r < M; r++) bc[r] = g.Cs[blk(n) + j + M * r];
+    bd = g.ds[vec(n) + j];
+  };
+  if (cnt > 0) load_back(rowof(cnt - 1));
+  for (int s = cnt - 1; s >= 0; s--) {
+    const int n = rowof(s);
+    const T dj = bd;
+    if (lane_ok) {
+#pragma unroll
+      for (int r = 0; r < M; r++) sF[j + M * r] = bc[r];
+    }
+    wb();
+    if (s >= 1) load_back(rowof(s - 1));
+    T acc = dj;
+#pragma unroll
+    for (int k = 0; k < M; k++) acc -= sF[j * M + k] * un[k];
+    if (lane_ok) su[j] = acc;
+    wb();
+#pragma unroll
+    for (int k = 0; k < M; k++) un[k] = su[k];
+    wb();
+    if (store) g.u[vec(n) + j] = acc;
+  }
+}
+
 template <typename T, int M>
 constexpr size_t blk2_lds() {
   return (size_t)4 * (kWave / M) * blk2_tile<T, M>() * sizeof(T);
+}
+
+// the two-ended tiled form: 2M lanes per problem (half 0 = lanes 0..M-1 of
+// the group, half 1 = lanes M..2M-1), 64/(2M) problems per wave, one tile
+// per half
+template <typename T, int M, int TRANS>
+__global__ void __launch_bounds__(256, blk2_min_waves(M)) k_blk2_tw(BlkArgs<T> g) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int G2 = kWave / (2 * M);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x >> 6;
+  const int grp = lane / (2 * M);
+  const int in = lane - grp * 2 * M;
+  const int h = in / M, j = in - h * M;
+  const bool lane_ok = grp < G2;
+  const long long p = ((long long)blockIdx.x * 4 + wave) * G2 + grp;
+  const bool store = lane_ok && p < g.P;
+  // idle lanes past the last whole group read the last group's first tile,
+  // solve problem P-1 again and write nothing
+  T *tile = reinterpret_cast<T *>(smem_raw) +
+            ((wave * G2 + (lane_ok ? grp : G2 - 1)) * 2 + (lane_ok ? h : 0)) * blk2_tile<T, M>();
+  blk2_problem_tw<T, M, TRANS>(g, std::min<long long>(p, g.P - 1), store, lane_ok, h, j,
+                               min(grp * 2 * M + h * M, kWave - 1),
+                               lane_ok ? grp * 2 * M + (1 - h) * M + j : lane, tile);
+}
+
+template <typename T, int M>
+constexpr size_t blk2_tw_lds() {
+  return (size_t)4 * (kWave / (2 * M)) * 2 * blk2_tile<T, M>() * sizeof(T);
 }
 
 // ---- mixed block sizes in one launch ---------------------------------------
@@ -964,16 +1207,24 @@ namespace {
 std::atomic<int> g_force_lanes{0};
 
 // layout: kTiled = the tiled kernel (k_blk2, M = 3..6), L > 0 = k_blk with
-// L lanes per problem, kTwist + L = k_blk_tw with L lanes per half, 0 = the
-// mixed launch's layout for M
+// L lanes per problem, kTwist + L = k_blk_tw with L lanes per half,
+// kTwistTiled = k_blk2_tw (M = 3..6), 0 = the mixed launch's layout for M
 constexpr int kTiled = -1;
 constexpr int kTwist = 100;
+constexpr int kTwistTiled = 200;
 
 template <typename T, int M, int TRANS>
 hipError_t launch_m(const BlkArgs<T> &g, int lanes, hipStream_t st) {
   constexpr int Ld = default_lanes(M);
   const int L = lanes > 0 ? lanes : Ld;
   if constexpr (blk2_sizes(M)) {
+    if (lanes == kTwistTiled) {
+      const long long G = 4 * (kWave / (2 * M));
+      const size_t lds = blk2_tw_lds<T, M>();
+      hipLaunchKernelGGL((k_blk2_tw<T, M, TRANS>), dim3((unsigned)((g.P + G - 1) / G)), dim3(256),
+                         lds, st, g);
+      return hipGetLastError();
+    }
     // one lane per block column, tiles through LDS
     if (lanes == 0 || lanes == kTiled) {
       const long long G = 4 * (kWave / M);
@@ -1039,6 +1290,11 @@ tridStatus_t check_blk(const void *A, const void *B, const void *C, const void *
   if (N > 0 && P > 0 && (!B || !d || !u || (N > 1 && (!A || !C))))
     return fail(TRID_STATUS_INVALID_VALUE, "NULL array");
   const int L = g_force_lanes.load();
+  if (L == kTwistTiled) {
+    if (!blk2_sizes(M))
+      return fail(TRID_STATUS_INVALID_VALUE, "the two-ended tiled layout needs blkdim 3..6");
+    return TRID_STATUS_SUCCESS;
+  }
   const int Lh = L > kTwist ? L - kTwist : L;  // lanes per half of the two-ended form
   if (L && M % Lh != 0) return fail(TRID_STATUS_INVALID_VALUE, "forced lanes do not divide blkdim");
   return TRID_STATUS_SUCCESS;
@@ -1046,21 +1302,21 @@ tridStatus_t check_blk(const void *A, const void *B, const void *C, const void *
 
 // Layout of a batch without tuning, from the tuner's sweeps on MI355X at
 // N = 128 over P = 4096 / 16384 / 65536 (one column each below: P < 8192,
-// P < 65536, larger), M = 2..6, 8, FP64 and FP32.  The two-ended form wins
-// most of them (FP64, ms of the tuning runs: M = 2, P = 4096: 0.201 against
-// 0.370 with one lane; M = 8, P = 16384: 3.73 against 4.08 with four lanes;
-// M = 3, P = 65536: 2.33 against 2.59; FP32 M = 5, P = 65536: 3.01 against
-// 4.02 tiled); the tiled kernel keeps the mid sizes of FP64 M = 3..5 (M = 4,
-// P = 16384: 0.964 against 1.054 two-ended) and one lane FP64 M = 2 at
-// P = 65536 (1.038 against 1.063).  0: not swept, the tiled kernel where it
-// exists, else default_lanes.
+// P < 65536, larger), M = 2..6, 8, FP64 and FP32.  A two-ended form wins
+// almost all of them (FP64, ms of the tuning runs: M = 2, P = 4096: 0.201
+// against 0.370 with one lane; M = 4, P = 65536: 3.82 two-ended tiled
+// against 4.00 tiled; M = 5, P = 4096: 0.571 against 1.089 tiled; M = 3,
+// P = 65536: 2.32 with one lane per half against 2.59 with one lane; FP32
+// M = 5, P = 65536: 3.00 against 4.05 tiled); one lane per problem keeps
+// FP64 M = 2 at P = 65536 (1.038 against 1.063).  0: not swept, the tiled
+// kernel where it exists, else default_lanes.
 int default_layout(int M, long long P, int ts) {
-  constexpr int T = kTiled, W1 = kTwist + 1, W2 = kTwist + 2, W4 = kTwist + 4;
-  static constexpr int k64[9][3] = {{0, 0, 0},   {0, 0, 0},    {W1, W1, 1}, {T, T, W1},
-                                    {W4, T, T},  {T, T, W1},   {T, W2, W2}, {0, 0, 0},
+  constexpr int W1 = kTwist + 1, W2 = kTwist + 2, W4 = kTwist + 4, TT = kTwistTiled;
+  static constexpr int k64[9][3] = {{0, 0, 0},    {0, 0, 0},    {W1, W1, 1}, {TT, TT, W1},
+                                    {TT, TT, TT}, {TT, TT, W1}, {TT, W2, TT}, {0, 0, 0},
                                     {W4, W4, W4}};
-  static constexpr int k32[9][3] = {{0, 0, 0},   {0, 0, 0},     {W2, W2, W1}, {W1, W1, W1},
-                                    {W4, W4, W1}, {W1, W1, W1}, {T, W2, W1},  {0, 0, 0},
+  static constexpr int k32[9][3] = {{0, 0, 0},    {0, 0, 0},    {W2, W2, W1}, {W1, W1, W1},
+                                    {TT, TT, W1}, {TT, W1, W1}, {TT, W2, W1}, {0, 0, 0},
                                     {W4, W4, W1}};
   const int band = P < 8192 ? 0 : (P < 65536 ? 1 : 2);
   const int t = M <= 8 ? (ts == 8 ? k64 : k32)[M][band] : 0;
@@ -1097,9 +1353,10 @@ int choose_layout(const BlkArgs<T> &g, int M, hipStream_t st) {
   if (blk2_sizes(M) && def != kTiled) cands.push_back(kTiled);
   for (int L = 1; L <= M && L <= 8; L++)
     if (M % L == 0 && L != def) cands.push_back(L);
-  // the two-ended form with 1, 2 or 4 lanes per half
+  // the two-ended forms: 1, 2 or 4 lanes per half, and tiled
   for (int L : {1, 2, 4})
     if (M % L == 0 && kTwist + L != def) cands.push_back(kTwist + L);
+  if (blk2_sizes(M) && def != kTwistTiled) cands.push_back(kTwistTiled);
   std::vector<float> tm;
   auto run = [&](int i) { return launch_blk<T, TRANS>(g, M, cands[i], st); };
   const int i = pick_fastest((int)cands.size(), [](int) { return true; }, run, st, &tm, 3, 2);
@@ -1248,8 +1505,8 @@ EXTERN_C tridStatus_t tridDbtsvAdjoint(const double *A, const double *B,
 
 EXTERN_C tridStatus_t tridBlkForceLanes(int lanes) {
   if ((lanes < 0 || lanes > 10) && lanes != kTwist + 1 && lanes != kTwist + 2 &&
-      lanes != kTwist + 4)
-    return fail(TRID_STATUS_INVALID_VALUE, "lanes must be in [0, 10] or 101, 102, 104");
+      lanes != kTwist + 4 && lanes != kTwistTiled)
+    return fail(TRID_STATUS_INVALID_VALUE, "lanes must be in [0, 10] or 101, 102, 104, 200");
   g_force_lanes.store(lanes);
   return TRID_STATUS_SUCCESS;
 }
