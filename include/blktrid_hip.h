@@ -10,7 +10,7 @@
  * (n*P + p)*M*M, vector entry (n, p) at (n*P + p)*M.  Cstar (N*P*M*M) and
  * dstar (N*P*M) are caller-provided scratch that receives the elimination's
  * intermediates: in the one-ended layouts the downward sweep's
- * C'_n = X_n^{-1} C_n (n < N-1) and d'_n (every n); in the two-ended one
+ * C'_n = X_n^{-1} C_n (n < N-1) and d'_n (every n); in the two-ended ones
  * (the default for most block sizes, see tridBlkForceLanes) C'_n and d'_n
  * at rows 0 .. N/2-1 and the upward sweep's A''_n = X_n^{-1} A_n and d''_n
  * at rows N/2+1 .. N-1.  The solution is u either way.  All pointers are
@@ -85,7 +85,7 @@ EXTERN_C tridStatus_t tridDbtsvAdjoint(const double *A, const double *B,
  * two-ended elimination with L lanes per half: one half eliminates
  * downward to row N/2, the other upward, two dependency chains of N/2 rows
  * per problem; 200 for the two-ended elimination on the tiled kernel, M
- * lanes per half, blkdim 3..6) -- for tests. */
+ * lanes per half, blkdim 3..8) -- for tests. */
 EXTERN_C tridStatus_t tridBlkForceLanes(int lanes);
 
 #endif

@@ -83,7 +83,7 @@ def test_block_two_ended_layouts(M, L, N):
     assert abs(lhs - rhs) <= 1e-10 * max(1.0, abs(lhs)), (M, L, N, lhs, rhs)
 
 
-@pytest.mark.parametrize("M", [3, 4, 5, 6])
+@pytest.mark.parametrize("M", [3, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("N", [1, 2, 3, 8, 17])
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 def test_block_two_ended_tiled(M, N, dt):
@@ -126,7 +126,7 @@ def test_block_force_lanes_rejects():
         blk.force_lanes(0)
     M = 2
     A, B, C, d = blk_system(M, N, P, seed=2)
-    blk.force_lanes(200)  # the tiled kernels take M = 3..6
+    blk.force_lanes(200)  # the two-ended tiled kernel takes M = 3..8
     try:
         with pytest.raises(ta.TridError):
             blk.btsv(*(dev(x) for x in (A, B, C, d)))

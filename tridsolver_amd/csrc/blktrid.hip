@@ -871,6 +871,9 @@ __global__ void __launch_bounds__(256, blk2_min_waves(M)) k_blk2(BlkArgs<T> g) {
 // MI355X; for M = 1, 2 the direct kernel's occupancy wins, for M = 7, 8 the
 // tiled kernel's register demand does not fit).
 __host__ __device__ constexpr bool blk2_sizes(int M) { return M >= 3 && M <= 6; }
+// Block sizes of the two-ended tiled kernel (half the live set of k_blk2 per
+// lane: no spills up to M = 8, 200 VGPRs at M = 8 FP64).
+__host__ __device__ constexpr bool blk2tw_sizes(int M) { return M >= 3 && M <= 8; }
 
 // Two-ended form of blk2_problem (the elimination of blk_problem_tw with the
 // tiled kernel's cooperative block moves): two groups of M lanes per
@@ -1208,7 +1211,7 @@ std::atomic<int> g_force_lanes{0};
 
 // layout: kTiled = the tiled kernel (k_blk2, M = 3..6), L > 0 = k_blk with
 // L lanes per problem, kTwist + L = k_blk_tw with L lanes per half,
-// kTwistTiled = k_blk2_tw (M = 3..6), 0 = the mixed launch's layout for M
+// kTwistTiled = k_blk2_tw (M = 3..8), 0 = the mixed launch's layout for M
 constexpr int kTiled = -1;
 constexpr int kTwist = 100;
 constexpr int kTwistTiled = 200;
@@ -1217,7 +1220,7 @@ template <typename T, int M, int TRANS>
 hipError_t launch_m(const BlkArgs<T> &g, int lanes, hipStream_t st) {
   constexpr int Ld = default_lanes(M);
   const int L = lanes > 0 ? lanes : Ld;
-  if constexpr (blk2_sizes(M)) {
+  if constexpr (blk2tw_sizes(M)) {
     if (lanes == kTwistTiled) {
       const long long G = 4 * (kWave / (2 * M));
       const size_t lds = blk2_tw_lds<T, M>();
@@ -1225,6 +1228,8 @@ hipError_t launch_m(const BlkArgs<T> &g, int lanes, hipStream_t st) {
                          lds, st, g);
       return hipGetLastError();
     }
+  }
+  if constexpr (blk2_sizes(M)) {
     // one lane per block column, tiles through LDS
     if (lanes == 0 || lanes == kTiled) {
       const long long G = 4 * (kWave / M);
@@ -1291,8 +1296,8 @@ tridStatus_t check_blk(const void *A, const void *B, const void *C, const void *
     return fail(TRID_STATUS_INVALID_VALUE, "NULL array");
   const int L = g_force_lanes.load();
   if (L == kTwistTiled) {
-    if (!blk2_sizes(M))
-      return fail(TRID_STATUS_INVALID_VALUE, "the two-ended tiled layout needs blkdim 3..6");
+    if (!blk2tw_sizes(M))
+      return fail(TRID_STATUS_INVALID_VALUE, "the two-ended tiled layout needs blkdim 3..8");
     return TRID_STATUS_SUCCESS;
   }
   const int Lh = L > kTwist ? L - kTwist : L;  // lanes per half of the two-ended form
@@ -1302,24 +1307,27 @@ tridStatus_t check_blk(const void *A, const void *B, const void *C, const void *
 
 // Layout of a batch without tuning, from the tuner's sweeps on MI355X at
 // N = 128 over P = 4096 / 16384 / 65536 (one column each below: P < 8192,
-// P < 65536, larger), M = 2..6, 8, FP64 and FP32.  A two-ended form wins
+// P < 65536, larger), M = 2..10, FP64 and FP32.  A two-ended form wins
 // almost all of them (FP64, ms of the tuning runs: M = 2, P = 4096: 0.201
 // against 0.370 with one lane; M = 4, P = 65536: 3.82 two-ended tiled
 // against 4.00 tiled; M = 5, P = 4096: 0.571 against 1.089 tiled; M = 3,
 // P = 65536: 2.32 with one lane per half against 2.59 with one lane; FP32
-// M = 5, P = 65536: 3.00 against 4.05 tiled); one lane per problem keeps
-// FP64 M = 2 at P = 65536 (1.038 against 1.063).  0: not swept, the tiled
-// kernel where it exists, else default_lanes.
+// M = 5, P = 65536: 3.00 against 4.05 tiled; M = 8, P = 65536: 13.6
+// two-ended tiled against 15.1 with four lanes per half; M = 7, P = 65536:
+// 11.9 against 16.6 with seven lanes); one lane per problem keeps FP64
+// M = 2 at P = 65536 (1.038 against 1.063), M lanes FP64 M = 9, 10 (27.2 /
+// 32.0 ms; the two-ended tiled kernel stops at M = 8).  0: M lanes (M = 9,
+// 10) or default_lanes.
 int default_layout(int M, long long P, int ts) {
   constexpr int W1 = kTwist + 1, W2 = kTwist + 2, W4 = kTwist + 4, TT = kTwistTiled;
-  static constexpr int k64[9][3] = {{0, 0, 0},    {0, 0, 0},    {W1, W1, 1}, {TT, TT, W1},
-                                    {TT, TT, TT}, {TT, TT, W1}, {TT, W2, TT}, {0, 0, 0},
-                                    {W4, W4, W4}};
-  static constexpr int k32[9][3] = {{0, 0, 0},    {0, 0, 0},    {W2, W2, W1}, {W1, W1, W1},
-                                    {TT, TT, W1}, {TT, W1, W1}, {TT, W2, W1}, {0, 0, 0},
-                                    {W4, W4, W1}};
+  static constexpr int k64[11][3] = {{0, 0, 0},    {0, 0, 0},    {W1, W1, 1},  {TT, TT, W1},
+                                     {TT, TT, TT}, {TT, TT, W1}, {TT, W2, TT}, {TT, TT, TT},
+                                     {TT, TT, TT}, {0, 0, 0},    {0, 0, 0}};
+  static constexpr int k32[11][3] = {{0, 0, 0},    {0, 0, 0},    {W2, W2, W1}, {W1, W1, W1},
+                                     {TT, TT, W1}, {TT, W1, W1}, {TT, W2, W1}, {TT, TT, W1},
+                                     {TT, TT, TT}, {0, 0, W1},   {0, W2, W2}};
   const int band = P < 8192 ? 0 : (P < 65536 ? 1 : 2);
-  const int t = M <= 8 ? (ts == 8 ? k64 : k32)[M][band] : 0;
+  const int t = M >= 1 && M <= 10 ? (ts == 8 ? k64 : k32)[M][band] : 0;
   if (t) return t;
   return blk2_sizes(M) ? kTiled : default_lanes(M);
 }
@@ -1356,7 +1364,7 @@ int choose_layout(const BlkArgs<T> &g, int M, hipStream_t st) {
   // the two-ended forms: 1, 2 or 4 lanes per half, and tiled
   for (int L : {1, 2, 4})
     if (M % L == 0 && kTwist + L != def) cands.push_back(kTwist + L);
-  if (blk2_sizes(M) && def != kTwistTiled) cands.push_back(kTwistTiled);
+  if (blk2tw_sizes(M) && def != kTwistTiled) cands.push_back(kTwistTiled);
   std::vector<float> tm;
   auto run = [&](int i) { return launch_blk<T, TRANS>(g, M, cands[i], st); };
   const int i = pick_fastest((int)cands.size(), [](int) { return true; }, run, st, &tm, 3, 2);
