@@ -874,6 +874,8 @@ __host__ __device__ constexpr bool blk2_sizes(int M) { return M >= 3 && M <= 6; 
 // Block sizes of the two-ended tiled kernel (half the live set of k_blk2 per
 // lane: no spills up to M = 8, 200 VGPRs at M = 8 FP64).
 __host__ __device__ constexpr bool blk2tw_sizes(int M) { return M >= 3 && M <= 8; }
+// layout code of the two-ended tiled kernel (tridBlkForceLanes, MixedArgs::L)
+constexpr int kTwistTiled = 200;
 
 // Two-ended form of blk2_problem (the elimination of blk_problem_tw with the
 // tiled kernel's cooperative block moves): two groups of M lanes per
@@ -1096,10 +1098,9 @@ constexpr size_t blk2_lds() {
 
 // the two-ended tiled form: 2M lanes per problem (half 0 = lanes 0..M-1 of
 // the group, half 1 = lanes M..2M-1), 64/(2M) problems per wave, one tile
-// per half
+// per half; work-group `block` of 256 threads
 template <typename T, int M, int TRANS>
-__global__ void __launch_bounds__(256, blk2_min_waves(M)) k_blk2_tw(BlkArgs<T> g) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
+__device__ __forceinline__ void blk2_tw_block(const BlkArgs<T> &g, long long block, T *smem) {
   constexpr int G2 = kWave / (2 * M);
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x >> 6;
@@ -1107,15 +1108,21 @@ __global__ void __launch_bounds__(256, blk2_min_waves(M)) k_blk2_tw(BlkArgs<T> g
   const int in = lane - grp * 2 * M;
   const int h = in / M, j = in - h * M;
   const bool lane_ok = grp < G2;
-  const long long p = ((long long)blockIdx.x * 4 + wave) * G2 + grp;
+  const long long p = (block * 4 + wave) * G2 + grp;
   const bool store = lane_ok && p < g.P;
   // idle lanes past the last whole group read the last group's first tile,
   // solve problem P-1 again and write nothing
-  T *tile = reinterpret_cast<T *>(smem_raw) +
-            ((wave * G2 + (lane_ok ? grp : G2 - 1)) * 2 + (lane_ok ? h : 0)) * blk2_tile<T, M>();
+  T *tile = smem + ((wave * G2 + (lane_ok ? grp : G2 - 1)) * 2 + (lane_ok ? h : 0)) *
+                       blk2_tile<T, M>();
   blk2_problem_tw<T, M, TRANS>(g, std::min<long long>(p, g.P - 1), store, lane_ok, h, j,
                                min(grp * 2 * M + h * M, kWave - 1),
                                lane_ok ? grp * 2 * M + (1 - h) * M + j : lane, tile);
+}
+
+template <typename T, int M, int TRANS>
+__global__ void __launch_bounds__(256, blk2_min_waves(M)) k_blk2_tw(BlkArgs<T> g) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  blk2_tw_block<T, M, TRANS>(g, blockIdx.x, reinterpret_cast<T *>(smem_raw));
 }
 
 template <typename T, int M>
@@ -1128,7 +1135,7 @@ constexpr int kMaxGroups = 16;
 struct MixedArgs {
   BlkArgs<double> g[kMaxGroups];
   int M[kMaxGroups];
-  int L[kMaxGroups];  // lanes per problem (k_blk), or M for the tiled kernel
+  int L[kMaxGroups];  // lanes per problem (k_blk), M for the tiled kernel, or kTwistTiled
   long long first_block[kMaxGroups + 1];
   int ngroups;
 };
@@ -1157,6 +1164,12 @@ __global__ void __launch_bounds__(256) k_blk_mixed(MixedArgs a) {
     if constexpr (MM == 8) {                                                      \
       if (a.L[gi] == 4) {                                                         \
         blk_block<double, MM, 4, 0>(a.g[gi], lb);                                 \
+        break;                                                                    \
+      }                                                                           \
+    }                                                                             \
+    if constexpr (blk2tw_sizes(MM)) {                                             \
+      if (a.L[gi] == kTwistTiled) {                                               \
+        blk2_tw_block<double, MM, 0>(a.g[gi], lb, smem);                          \
         break;                                                                    \
       }                                                                           \
     }                                                                             \
@@ -1214,7 +1227,6 @@ std::atomic<int> g_force_lanes{0};
 // kTwistTiled = k_blk2_tw (M = 3..8), 0 = the mixed launch's layout for M
 constexpr int kTiled = -1;
 constexpr int kTwist = 100;
-constexpr int kTwistTiled = 200;
 
 template <typename T, int M, int TRANS>
 hipError_t launch_m(const BlkArgs<T> &g, int lanes, hipStream_t st) {
@@ -1452,13 +1464,15 @@ EXTERN_C tridStatus_t tridDbtsvMixedBatch(const tridDbtsvGroup_t *groups,
     a.g[k] = BlkArgs<double>{G.A, G.B, G.C, G.d, G.Cstar, G.dstar, G.u, G.N, G.P};
     a.M[k] = G.M;
     a.first_block[k] = nb;
-    // one lane per problem for M <= 3 and four for M = 8 in batches that
-    // fill the chip with them (the one-ended layouts that beat M lanes at
+    // the two-ended tiled layout where it is the untuned default; else one
+    // lane per problem for M <= 3 and four for M = 8 in batches that fill
+    // the chip with them (the one-ended layouts that beat M lanes at
     // P = 65536), else default_lanes
     const bool big = G.P >= 65536;
-    const int L = big && G.M <= 3 ? 1 : (big && G.M == 8 ? 4 : default_lanes(G.M));
+    int L = big && G.M <= 3 ? 1 : (big && G.M == 8 ? 4 : default_lanes(G.M));
+    if (default_layout(G.M, G.P, 8) == kTwistTiled) L = kTwistTiled;
     a.L[k] = L;
-    const long long per = 4LL * (kWave / L);
+    const long long per = L == kTwistTiled ? 4LL * (kWave / (2 * G.M)) : 4LL * (kWave / L);
     nb += (G.P + per - 1) / per;
     k++;
   }
@@ -1468,9 +1482,11 @@ EXTERN_C tridStatus_t tridDbtsvMixedBatch(const tridDbtsvGroup_t *groups,
   // LDS: the largest tile set of the batch's block sizes
   size_t lds = 0;
   for (int i = 0; i < k; i++) {
-    if (!blk2_sizes(a.M[i])) continue;
-    const size_t b = (size_t)4 * (kWave / a.M[i]) * (3 * a.M[i] * a.M[i] + 2 * a.M[i]) * 8;
-    lds = std::max(lds, b);
+    const int Mi = a.M[i], tile = 3 * Mi * Mi + 2 * Mi;
+    if (a.L[i] == kTwistTiled)
+      lds = std::max(lds, (size_t)4 * (kWave / (2 * Mi)) * 2 * tile * 8);
+    else if (blk2_sizes(Mi))
+      lds = std::max(lds, (size_t)4 * (kWave / Mi) * tile * 8);
   }
   hipStream_t st = current_stream();
   hipLaunchKernelGGL(k_blk_mixed, dim3((unsigned)nb), dim3(256), lds, st, a);
