@@ -249,17 +249,27 @@ __global__ void __launch_bounds__(256)
 // thread per point) and the neighbour planes never come back from L2 or HBM.
 // Every load of a point is issued before its first use; neighbours off the
 // grid read the point itself and are masked.
+// Tiles are taken in XCD-contiguous order: work-group b of the 1-D grid (the
+// hardware deals work-groups to the 8 XCDs round-robin) takes tile
+// (b mod 8) * (B / 8) + b / 8 of the x-fastest, then y, then z tile order (B
+// the grid rounded down to a multiple of 8), so that the tiles of one XCD
+// are y-neighbours and find each other's halo rows in that XCD's L2.
 template <int ZC>
 __global__ void __launch_bounds__(256)
     k_vadi_grad_col(const double *__restrict__ K, const double *__restrict__ U,
                     const double *__restrict__ U1, const double *__restrict__ P,
                     const double *__restrict__ Q, const double *__restrict__ LZ,
                     const double *__restrict__ LY, const double *__restrict__ LX,
-                    double *__restrict__ UB, double *__restrict__ GK, Grid g, double lam) {
-  const int x = blockIdx.x * 64 + threadIdx.x;
-  const int y = blockIdx.y * 4 + threadIdx.y;
+                    double *__restrict__ UB, double *__restrict__ GK, Grid g, double lam,
+                    int ntx, int nty) {
+  const unsigned nb8 = gridDim.x & ~7u, b = blockIdx.x;
+  const unsigned t = b < nb8 ? (b & 7u) * (nb8 >> 3) + (b >> 3) : b;
+  const int tx = (int)(t % (unsigned)ntx), tyz = (int)(t / (unsigned)ntx);
+  const int ty = tyz % nty, tz = tyz / nty;
+  const int x = tx * 64 + threadIdx.x;
+  const int y = ty * 4 + threadIdx.y;
   if (x >= g.nx || y >= g.ny) return;
-  const int z0 = blockIdx.z * ZC;
+  const int z0 = tz * ZC;
   const int z1 = z0 + ZC < g.nz ? z0 + ZC : g.nz;
   const long long col = (long long)y * g.nx + x;
   // plane values: U, K, LX, U1, LZ
@@ -641,8 +651,9 @@ EXTERN_C tridStatus_t tridVadiLossGrad(const double *kappa, const double *u0,
   s = check_hip(hipGetLastError(), "loss launch");
   if (s) return s;
 
-  const dim3 grad_grid((unsigned)((nx + 63) / 64), (unsigned)((ny + 3) / 4),
-                       (unsigned)((nz + kGradZ - 1) / kGradZ));
+  const int grad_tx = (nx + 63) / 64, grad_ty = (ny + 3) / 4;
+  const unsigned grad_grid =
+      (unsigned)grad_tx * (unsigned)grad_ty * (unsigned)((nz + kGradZ - 1) / kGradZ);
   for (int sg = nseg - 1; sg >= 0 && !s; sg--) {
     const int k0 = sg * ckpt, k1 = std::min(steps, k0 + ckpt);
     if (sg < nseg - 1) s = record(sg);
@@ -652,8 +663,8 @@ EXTERN_C tridStatus_t tridVadiLossGrad(const double *kappa, const double *u0,
       if (!s) s = vadi_sweep(kappa, LZ, nullptr, LY, g, 1, lam, 1, 0, st);
       if (!s) s = vadi_xT(kappa, LY, LX, g, lam, st);
       if (s) break;
-      hipLaunchKernelGGL(k_vadi_grad_col<kGradZ>, grad_grid, dim3(64, 4), 0, st, kappa, us[i],
-                         us[i + 1], P[i], Q[i], LZ, LY, LX, ubar, grad_kappa, g, lam);
+      hipLaunchKernelGGL(k_vadi_grad_col<kGradZ>, dim3(grad_grid), dim3(64, 4), 0, st, kappa, us[i],
+                         us[i + 1], P[i], Q[i], LZ, LY, LX, ubar, grad_kappa, g, lam, grad_tx, grad_ty);
       s = check_hip(hipGetLastError(), "vadi gradient launch");
     }
   }
